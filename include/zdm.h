@@ -1,0 +1,105 @@
+/*
+ * zdm.h -- C ABI of the MI355X (gfx950) finite-volume time step for SME / VAM models.
+ *
+ * One shared library per model (libzdm_<model>.so), built from the model's generated
+ * Model.H / Numerics.H exactly as the reference builds solver_cpu on them (Makefile:46-55).
+ * Every entry point returns 0 on success and a negative code on failure; the message is
+ * available from zdm_last_error().  All device work is ordered on the solver's own HIP
+ * stream; a handle must not be used from two threads at once.
+ *
+ * State arrays at this boundary use the reference's layout: array-of-structs per cell,
+ * x[c*n_q + i] (the PETSc Vec layout of dmQ, VirtualSolver.hpp:447-457), cells in the
+ * mesh's solver order.  Device storage is structure-of-arrays and never leaves the handle.
+ */
+#ifndef ZDM_H
+#define ZDM_H
+
+#include "zdm_mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zdm_solver zdm_solver;
+
+/* Solver options: the subset of SolverSettings (Settings.hpp:13-40) the step consumes. */
+typedef struct zdm_options {
+    int order;          /* reconstruction_order: 1 = PCM + forward Euler, >= 2 = MUSCL + SSP-RK */
+    int ssp_stages;     /* s of the s-stage SSP-RK2 (TSSSP "rks2", SolverStrategies.hpp:33-35) */
+    int use_limiters;   /* limiter != "none" -> Vacuum + TVD (MUSCLSolver.hpp:40-43) */
+    int gradient;       /* 0 = Green-Gauss (reference, ModularSolver.hpp:62), 1 = least squares */
+    int source;         /* time_integration == "splitting": split implicit source (MUSCLSolver.hpp:179) */
+    double cfl;         /* solver.cfl */
+    double t_end;       /* solver.t_end */
+    double min_dt;      /* solver.min_dt (first step only, MUSCLSolver.hpp:103) */
+    int block_size;     /* workgroup size (0 = 256) */
+    int well_balanced;  /* extension: add Model::hydrostatic_cell_source per face side (second-order
+                           hydrostatic reconstruction); 0 = the reference scheme */
+} zdm_options;
+
+/* Model plug-in description (Model.H:31-44). */
+int zdm_model_info(int* n_q, int* n_aux, int* n_params, int* dim);
+const char* zdm_model_name(void);
+int zdm_model_default_params(double* params);              /* Model::default_parameters() */
+/* Model::get_boundary_tags(): writes up to cap names of 64 chars, returns the count. */
+int zdm_model_boundary_tags(char* names64, int cap);
+/* Model::parameter_names() (Model.H:43), same layout. */
+int zdm_model_parameter_names(char* names64, int cap);
+
+/* Create a solver on `mesh`.  face_bc[f] = model BC index of boundary face f, or -1 when its
+ * tag maps to no model BC (such faces get no flux, TransportStep.hpp:498).  Replaces
+ * VirtualSolver::SetupArchitecture (VirtualSolver.hpp:386-525) + InitializeComponents
+ * (ModularSolver.hpp:39-76). */
+int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* params,
+                      const zdm_options* opts, zdm_solver** out);
+int zdm_solver_destroy(zdm_solver* s);
+
+/* Host <-> device state transfer (AoS, n_q / n_aux per cell).  A may be NULL (aux = 0). */
+int zdm_state_upload(zdm_solver* s, const double* X, const double* A);
+int zdm_state_download(zdm_solver* s, double* X, double* A);
+
+/* EnforcePhysicalConstraints(X) + the first dt = max(ComputeTimeStep(), min_dt)
+ * (MUSCLSolver.hpp:54, 103).  Resets time and step count to 0. */
+int zdm_initialize(zdm_solver* s);
+
+/* Enqueue n full time steps of the MUSCLSolver loop (MUSCLSolver.hpp:140-188): clamp dt to
+ * t_end, TSStep (SSP-RK / Euler), constraints, split source, constraints, CFL for the next
+ * dt.  Steps past t_end are no-ops.  Asynchronous: no host synchronisation. */
+int zdm_step(zdm_solver* s, int n);
+
+/* Run until t_end (or max_steps); synchronises every `check_every` steps. */
+int zdm_run(zdm_solver* s, int max_steps, int check_every, int* steps_done);
+
+/* Time, current/next dt and step count (synchronises). */
+int zdm_get_time(zdm_solver* s, double* time, double* dt, int* steps);
+int zdm_synchronize(zdm_solver* s);
+
+/* One transport right-hand side F = L(X) at time t (TransportStep::FormRHS,
+ * TransportStep.hpp:325-357) on host arrays; A is the scratch aux input (may be NULL). */
+int zdm_transport_rhs(zdm_solver* s, double t, const double* X, const double* A, double* F);
+
+/* SourceStep::Solve(dt, X, A) on the resident state (SourceStep.hpp:72-122). */
+int zdm_source_step(zdm_solver* s, double dt);
+
+/* MUSCLSolver::EnforcePhysicalConstraints on the resident state (MUSCLSolver.hpp:234-254). */
+int zdm_enforce_constraints(zdm_solver* s);
+
+/* VirtualSolver::ComputeTimeStep on the resident state (VirtualSolver.hpp:285-357). */
+int zdm_cfl_dt(zdm_solver* s, double* dt);
+
+/* Device-pointer variants (AoS device buffers, ordered on the solver stream). */
+int zdm_state_upload_device(zdm_solver* s, const double* dX, const double* dA);
+int zdm_state_download_device(zdm_solver* s, double* dX, double* dA);
+
+/* Per-kernel HIP-event timing of zdm_step launches while enabled (ms, averaged per launch;
+ * events are resolved without synchronising inside zdm_step).  Kernel ids:
+ * 0 = prep (gradient/bounds/reconstruction), 1 = flux + stage update, 2 = post-step. */
+int zdm_enable_timing(zdm_solver* s, int on);
+int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches);
+
+const char* zdm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,93 @@
+/*
+ * zdm_mesh.h -- host-side finite-volume mesh service (C ABI).
+ *
+ * Replaces the DMPlex services the FV time step consumes:
+ *   - cell/face connectivity and the cone -> support adjacency
+ *     (reference VirtualSolver.hpp:386-511, DMPlexDistribute + DMSetBasicAdjacency at :439);
+ *   - FV geometry (DMPlexGetGeometryFVM): cell centroid + volume, face centroid and the
+ *     area-weighted normal oriented from support[0] to support[1] (outward on the boundary)
+ *     (consumed at TransportStep.hpp:405-408, 434-435, 556-559; Gradient.hpp:92-93);
+ *   - the "Face Sets" label (boundary tags) from Gmsh $PhysicalNames
+ *     (MeshConfigLoader.hpp:15-41, VirtualSolver.hpp:394-436).
+ *
+ * Cells are renumbered by reverse Cuthill-McKee (optional) so that face neighbours are
+ * close in memory; faces are numbered by ascending (support[0], support[1]).
+ * Every function returns 0 on success and a negative code on failure; the message is
+ * available from zdm_mesh_last_error().
+ */
+#ifndef ZDM_MESH_H
+#define ZDM_MESH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZDM_MESH_MAX_FACES 4   /* segments: 2, triangles: 3, quadrilaterals: 4 */
+
+typedef struct zdm_mesh {
+    int dim;          /* 1 (segments) or 2 (triangles / quadrilaterals) */
+    int ncells;
+    int nfaces;
+    int max_faces;    /* max faces per cell in this mesh */
+    int nverts;
+    int max_cell_verts;
+
+    /* cells, in the (renumbered) solver order */
+    double* cell_centroid;   /* ncells*3 */
+    double* cell_volume;     /* ncells   */
+    int*    cell_nfaces;     /* ncells   */
+    int*    cell_faces;      /* ncells*max_faces, ascending face ids, -1 padded */
+    int*    cell_nbrs;       /* ncells*max_faces, cell across that face, -1 on the boundary */
+    int*    cell_perm;       /* ncells: solver cell -> input cell index */
+    int*    cell_verts;      /* ncells*max_cell_verts, -1 padded (solver order) */
+
+    /* faces, ordered by (support[0], support[1]) */
+    int*    face_cells;      /* nfaces*2: support[0] < support[1]; support[1] = -1 on the boundary */
+    double* face_normal;     /* nfaces*3: area-weighted normal, support[0] -> support[1] */
+    double* face_centroid;   /* nfaces*3 */
+    int*    face_tag;        /* nfaces: "Face Sets" value of boundary faces, -1 if untagged / interior */
+
+    double* verts;           /* nverts*3 */
+
+    /* Gmsh physical names: name i has tag physical_tag[i] and dimension physical_dim[i] */
+    int   nphysical;
+    int*  physical_tag;
+    int*  physical_dim;
+    char* physical_names;    /* nphysical entries of 64 chars, NUL terminated */
+} zdm_mesh;
+
+/* Build from an explicit cell list.  cell_verts is ncells*nvpc (-1 padded for mixed
+ * meshes).  Boundary face tags: nbf faces given by their vertices (1 vertex per face in 1-D,
+ * 2 in 2-D) with a tag each.  If mark_all_tag > 0, every boundary face without a tag gets
+ * it (DMPlexMarkBoundaryFaces, VirtualSolver.hpp:428-436).  renumber != 0 applies RCM. */
+int zdm_mesh_from_cells(int dim, int nverts, const double* xyz, int ncells, int nvpc,
+                        const int* cell_verts, int nbf, const int* bface_verts,
+                        const int* bface_tags, int mark_all_tag, int renumber,
+                        zdm_mesh** out);
+
+/* Unit square [0,1]^2 as an nx x ny grid of squares, each split into two right triangles
+ * along the SW -> NE diagonal.  Interior nodes are jittered by U(-jitter, jitter) times the
+ * edge length in each direction (SplitMix64 stream seeded with `seed`).  All boundary faces
+ * get tag 1. */
+int zdm_mesh_tri_grid(int nx, int ny, double jitter, uint64_t seed, int renumber,
+                      zdm_mesh** out);
+
+/* Uniform 1-D grid of n segments on [x0, x1]; left end tag 1, right end tag 2. */
+int zdm_mesh_line(int n, double x0, double x1, int renumber, zdm_mesh** out);
+
+/* Gmsh MSH 2.2 ASCII reader (points, lines, triangles, quadrilaterals). */
+int zdm_mesh_read_msh(const char* path, int renumber, zdm_mesh** out);
+
+void zdm_mesh_destroy(zdm_mesh* m);
+const char* zdm_mesh_last_error(void);
+
+/* SplitMix64: next value of the stream and a uniform double in [0,1). */
+uint64_t zdm_splitmix64_next(uint64_t* state);
+double   zdm_splitmix64_uniform(uint64_t* state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,661 @@
+// ORACLE -- test infrastructure only.  CPU restatement of the reference's finite-volume time
+// step (ZoomyLab/zoomy-dmplex), used by tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg as the checker.  Nothing in the product links or calls this file.
+//
+// It follows the reference's own loop structure (face loops that scatter into both cells,
+// array-of-structs state x[c*n_q + i], gradient g[(c*n_q + i)*dim + d]) so that it is an
+// independent restatement of the device path, which gathers per cell.  Each function cites
+// the reference lines it restates.  It is instantiated on a model directory (Model.H +
+// Numerics.H) given on the include path, exactly as the reference is built on its generated
+// headers.
+//
+// Pinning: see tests/test_oracle_*.py (closed forms, invariants, the reference's KATs).
+// Parity unpinned: PETSc's TSSSP RKS2 / VecAXPBYPCZ rounding order (third-party, restated
+// from its published algorithm), the DMPlex face numbering (the oracle uses the mesh
+// service's ordering).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include <type_traits>
+#include <utility>
+
+#include "Numerics.H"
+#include "zdm_mesh.h"
+
+template <typename TT, typename = void>
+struct HasCellSource : std::false_type {};
+template <typename TT>
+struct HasCellSource<TT, std::void_t<decltype(Model<TT>::hydrostatic_cell_source(
+                             std::declval<const TT*>(), std::declval<const TT*>(),
+                             std::declval<const TT*>(), std::declval<const TT*>()))>>
+    : std::true_type {};
+
+namespace {
+
+using T = double;
+using M = Model<T>;
+using NUM = Numerics<T>;
+constexpr int NQ = M::n_dof_q;
+constexpr int NA = M::n_dof_qaux;
+constexpr int DIM = M::dimension;
+
+struct Opts {
+    int order;            // 1 = PCM + forward Euler, 2 = MUSCL + SSP-RK
+    int ssp_stages;       // s of SSP-RK(s,2)
+    int use_limiters;     // Vacuum + TVD (ModularSolver.hpp:60, Reconstruction.hpp:170-175)
+    int gradient;         // 0 = Green-Gauss (reference default), 1 = least squares
+    int well_balanced;    // extension: Model::hydrostatic_cell_source per face side
+    double cfl;
+    double t_end;
+    double min_dt;
+};
+
+// ---------------------------------------------------------------- TransportStep pieces
+
+// TransportStep::UpdateState, TransportStep.hpp:196-230 (per cell clamp + local aux)
+void update_state(int nc, const T* p, T* X, T* A) {
+    const T X0[3] = {0, 0, 0};
+    for (int c = 0; c < nc; ++c) {
+        T* q = X + (size_t)c * NQ;
+        T* a = A + (size_t)c * NA;
+        auto rq = M::update_variables(q, a, p, 0.0);
+        for (int i = 0; i < NQ; ++i) q[i] = rq[i];
+        auto ra = M::update_aux_variables(q, a, p, 0.0, X0);
+        for (int i = 0; i < NA; ++i) a[i] = ra[i];
+    }
+}
+
+// TransportStep::UpdateNeighborBounds, TransportStep.hpp:122-154
+void neighbor_bounds(const zdm_mesh* m, const T* X, T* vmin, T* vmax) {
+    for (int c = 0; c < m->ncells; ++c) {
+        const T* q = X + (size_t)c * NQ;
+        T lo[NQ], hi[NQ];
+        for (int i = 0; i < NQ; ++i) lo[i] = hi[i] = q[i];
+        for (int k = 0; k < m->cell_nfaces[c]; ++k) {
+            int n = m->cell_nbrs[(size_t)c * m->max_faces + k];
+            if (n < 0) continue;
+            const T* qn = X + (size_t)n * NQ;
+            for (int i = 0; i < NQ; ++i) {
+                if (qn[i] < lo[i]) lo[i] = qn[i];
+                if (qn[i] > hi[i]) hi[i] = qn[i];
+            }
+        }
+        for (int i = 0; i < NQ; ++i) {
+            vmin[(size_t)c * NQ + i] = lo[i];
+            vmax[(size_t)c * NQ + i] = hi[i];
+        }
+    }
+}
+
+// GreenGaussGradient::Compute, Gradient.hpp:46-190 (boundary face value = the model's BC
+// ghost with Qaux = nullptr, Gradient.hpp:140-153 and ModularSolver.hpp:63-66)
+void gradient_gg(const zdm_mesh* m, const int* face_bc, const T* p, const T* X, T* G) {
+    std::memset(G, 0, sizeof(T) * (size_t)m->ncells * NQ * DIM);
+    for (int f = 0; f < m->nfaces; ++f) {
+        const T* N = m->face_normal + 3 * (size_t)f;
+        const int cL = m->face_cells[2 * f], cR = m->face_cells[2 * f + 1];
+        if (cR >= 0) {
+            const T* qL = X + (size_t)cL * NQ;
+            const T* qR = X + (size_t)cR * NQ;
+            T* gL = G + (size_t)cL * NQ * DIM;
+            T* gR = G + (size_t)cR * NQ * DIM;
+            for (int i = 0; i < NQ; ++i) {
+                T fv = 0.5 * (qL[i] + qR[i]);
+                for (int d = 0; d < DIM; ++d) {
+                    T v = fv * N[d];
+                    gL[i * DIM + d] += v;
+                    gR[i * DIM + d] -= v;
+                }
+            }
+        } else if (face_bc[f] >= 0) {
+            const T* qL = X + (size_t)cL * NQ;
+            T area = 0;
+            for (int d = 0; d < DIM; ++d) area += N[d] * N[d];
+            area = std::sqrt(area);
+            T nh[3] = {0, 0, 0};
+            for (int d = 0; d < DIM; ++d) nh[d] = N[d] / area;
+            auto qb = M::boundary_conditions(face_bc[f], qL, nullptr, p, nh,
+                                             m->face_centroid + 3 * (size_t)f, 0.0, 0.0);
+            T* gL = G + (size_t)cL * NQ * DIM;
+            for (int i = 0; i < NQ; ++i)
+                for (int d = 0; d < DIM; ++d) gL[i * DIM + d] += qb[i] * N[d];
+        }
+    }
+    for (int c = 0; c < m->ncells; ++c) {
+        T vol = m->cell_volume[c];
+        if (vol > 1e-15)
+            for (int k = 0; k < NQ * DIM; ++k) G[(size_t)c * NQ * DIM + k] /= vol;
+    }
+}
+
+// LeastSquaresGradient::SolveSmallSystemSPD, Gradient.hpp:205-217
+bool chol_solve(int n, std::vector<T>& A, std::vector<T>& b) {
+    std::vector<T> L(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j <= i; j++) {
+            T sum = 0;
+            for (int k = 0; k < j; k++) sum += L[i * n + k] * L[j * n + k];
+            if (i == j) {
+                T val = A[i * n + i] - sum;
+                if (val <= 0) return false;
+                L[i * n + j] = std::sqrt(val);
+            } else {
+                L[i * n + j] = (A[i * n + j] - sum) / L[j * n + j];
+            }
+        }
+    for (int i = 0; i < n; i++) {
+        T sum = 0;
+        for (int k = 0; k < i; k++) sum += L[i * n + k] * b[k];
+        b[i] = (b[i] - sum) / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        T sum = 0;
+        for (int k = i + 1; k < n; k++) sum += L[k * n + i] * b[k];
+        b[i] = (b[i] - sum) / L[i * n + i];
+    }
+    return true;
+}
+
+// LeastSquaresGradient::Setup + Compute (order 1), Gradient.hpp:222-298, 300-359
+void gradient_lsq(const zdm_mesh* m, const T* X, T* G) {
+    std::memset(G, 0, sizeof(T) * (size_t)m->ncells * NQ * DIM);
+    for (int c = 0; c < m->ncells; ++c) {
+        const T* xc = m->cell_centroid + 3 * (size_t)c;
+        std::vector<int> nb;
+        std::vector<std::vector<T>> bv;
+        for (int k = 0; k < m->cell_nfaces[c]; ++k) {
+            int n = m->cell_nbrs[(size_t)c * m->max_faces + k];
+            if (n < 0) continue;
+            const T* xn = m->cell_centroid + 3 * (size_t)n;
+            std::vector<T> b(DIM);
+            for (int d = 0; d < DIM; ++d) b[d] = xn[d] - xc[d];
+            nb.push_back(n);
+            bv.push_back(b);
+        }
+        std::vector<T> ata(DIM * DIM, 0.0);
+        for (const auto& b : bv)
+            for (int i = 0; i < DIM; ++i)
+                for (int j = 0; j < DIM; ++j) ata[i * DIM + j] += b[i] * b[j];
+        const T* qc = X + (size_t)c * NQ;
+        T* gc = G + (size_t)c * NQ * DIM;
+        for (size_t k = 0; k < nb.size(); ++k) {
+            std::vector<T> w = bv[k], sys = ata;
+            if (!chol_solve(DIM, sys, w)) std::fill(w.begin(), w.end(), 0.0);
+            const T* qn = X + (size_t)nb[k] * NQ;
+            for (int i = 0; i < NQ; ++i) {
+                T du = qn[i] - qc[i];
+                for (int d = 0; d < DIM; ++d) gc[i * DIM + d] += w[d] * du;
+            }
+        }
+    }
+}
+
+// LinearReconstructor::Reconstruct with VacuumLimiter + TVDLimiter,
+// Reconstruction.hpp:29-35, 64-80, 203-243; PCMReconstructor 147-149 when grad == nullptr
+void reconstruct(const T* q, const T* grad, const T* xc, const T* xf, const T* qmin, const T* qmax,
+                 int use_limiters, T* out) {
+    if (!grad) {
+        for (int i = 0; i < NQ; ++i) out[i] = q[i];
+        return;
+    }
+    T dx[3];
+    for (int d = 0; d < DIM; ++d) dx[d] = xf[d] - xc[d];
+    T unl[NQ], delta[NQ];
+    for (int i = 0; i < NQ; ++i) {
+        delta[i] = 0.0;
+        for (int d = 0; d < DIM; ++d) delta[i] += grad[i * DIM + d] * dx[d];
+        unl[i] = q[i] + delta[i];
+    }
+    T fin[NQ];
+    for (int i = 0; i < NQ; ++i) fin[i] = 1.0;
+    if (use_limiters) {
+        // vacuum
+        T tmp[NQ];
+        for (int i = 0; i < NQ; ++i) tmp[i] = 1.0;
+        if (NQ > 1 && qmin && qmin[1] < 1e-2)
+            for (int i = 0; i < NQ; ++i) tmp[i] = 0.0;
+        for (int i = 0; i < NQ; ++i)
+            if (tmp[i] < fin[i]) fin[i] = tmp[i];
+        // TVD (Barth-Jespersen, face-wise)
+        for (int i = 0; i < NQ; ++i) tmp[i] = 1.0;
+        if (qmin && qmax) {
+            for (int i = 0; i < NQ; ++i) {
+                T dq = unl[i] - q[i];
+                if (std::fabs(dq) < 1e-12) continue;
+                T a = 1.0;
+                if (dq > 0) {
+                    if (qmax[i] > q[i])
+                        a = std::min((T)1.0, (qmax[i] - q[i]) / dq);
+                    else
+                        a = 0.0;
+                } else {
+                    if (qmin[i] < q[i])
+                        a = std::min((T)1.0, (qmin[i] - q[i]) / dq);
+                    else
+                        a = 0.0;
+                }
+                tmp[i] = a;
+            }
+        }
+        for (int i = 0; i < NQ; ++i)
+            if (tmp[i] < fin[i]) fin[i] = tmp[i];
+    }
+    T alpha = 1.;
+    for (int i = 0; i < NQ; ++i)
+        if (alpha > fin[i]) alpha = fin[i];
+    for (int i = 0; i < NQ; ++i) out[i] = q[i] + alpha * delta[i];
+}
+
+// TransportStep::ComputeFluxes, TransportStep.hpp:383-565 (owned = all cells on one rank)
+// Extension (well_balanced): after a face's flux, each side adds
+// area * Model::hydrostatic_cell_source(q_face, q_cell, p, outward normal).
+void cell_source(int wb, const T* qf, const T* qc, const T* p, const T* nh, T sign, T area, T* f) {
+    if constexpr (HasCellSource<T>::value) {
+        if (!wb) return;
+        const T no[3] = {sign < 0 ? -nh[0] : nh[0], sign < 0 ? -nh[1] : nh[1], 0};
+        auto S = M::hydrostatic_cell_source(qf, qc, p, no);
+        for (int i = 0; i < NQ; ++i) f[i] += S[i] * area;
+    }
+}
+
+void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, const T* X,
+                    const T* A, const T* G, const T* vmin, const T* vmax, int use_limiters, int wb,
+                    T* F) {
+    for (int f = 0; f < m->nfaces; ++f) {
+        const T* N = m->face_normal + 3 * (size_t)f;
+        const T* xf = m->face_centroid + 3 * (size_t)f;
+        T area = 0;
+        for (int d = 0; d < DIM; ++d) area += N[d] * N[d];
+        area = std::sqrt(area);
+        if (area <= 1e-15) continue;
+        T nh[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; ++d) nh[d] = N[d] / area;
+        const int cL = m->face_cells[2 * f], cR = m->face_cells[2 * f + 1];
+        if (cR >= 0) {
+            T qLf[NQ], qRf[NQ], aLf[NA], aRf[NA];
+            reconstruct(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
+                        m->cell_centroid + 3 * (size_t)cL, xf, vmin ? vmin + (size_t)cL * NQ : nullptr,
+                        vmax ? vmax + (size_t)cL * NQ : nullptr, use_limiters, qLf);
+            reconstruct(X + (size_t)cR * NQ, G ? G + (size_t)cR * NQ * DIM : nullptr,
+                        m->cell_centroid + 3 * (size_t)cR, xf, vmin ? vmin + (size_t)cR * NQ : nullptr,
+                        vmax ? vmax + (size_t)cR * NQ : nullptr, use_limiters, qRf);
+            for (int i = 0; i < NA; ++i) {
+                aLf[i] = A[(size_t)cL * NA + i];
+                aRf[i] = A[(size_t)cR * NA + i];
+            }
+            auto raL = M::update_aux_variables(qLf, aLf, p, 0.0, xf);
+            for (int i = 0; i < NA; ++i) aLf[i] = raL[i];
+            auto raR = M::update_aux_variables(qRf, aRf, p, 0.0, xf);
+            for (int i = 0; i < NA; ++i) aRf[i] = raR[i];
+            auto flux = NUM::numerical_flux(qLf, qRf, aLf, aRf, p, nh);
+            T* fL = F + (size_t)cL * NQ;
+            T* fR = F + (size_t)cR * NQ;
+            auto nc = NUM::numerical_fluctuations(qLf, qRf, aLf, aRf, p, nh);
+            for (int i = 0; i < NQ; ++i) {
+                fL[i] -= nc[NQ + i] * area;
+                fR[i] -= nc[i] * area;
+            }
+            for (int i = 0; i < NQ; ++i) {
+                fL[i] -= flux[i] * area;
+                fR[i] += flux[i] * area;
+            }
+            cell_source(wb, qLf, X + (size_t)cL * NQ, p, nh, 1.0, area, fL);
+            cell_source(wb, qRf, X + (size_t)cR * NQ, p, nh, -1.0, area, fR);
+        } else if (face_bc[f] >= 0) {
+            T qLf[NQ], aLf[NA], aRf[NA];
+            reconstruct(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
+                        m->cell_centroid + 3 * (size_t)cL, xf, vmin ? vmin + (size_t)cL * NQ : nullptr,
+                        vmax ? vmax + (size_t)cL * NQ : nullptr, use_limiters, qLf);
+            for (int i = 0; i < NA; ++i) aLf[i] = A[(size_t)cL * NA + i];
+            auto raL = M::update_aux_variables(qLf, aLf, p, time, xf);
+            for (int i = 0; i < NA; ++i) aLf[i] = raL[i];
+            auto qR = M::boundary_conditions(face_bc[f], qLf, aLf, p, nh, xf, time, 0.0);
+            auto raR = M::update_aux_variables(qR.data, aLf, p, time, xf);
+            for (int i = 0; i < NA; ++i) aRf[i] = raR[i];
+            auto flux = NUM::numerical_flux(qLf, qR.data, aLf, aRf, p, nh);
+            T* fL = F + (size_t)cL * NQ;
+            auto nc = NUM::numerical_fluctuations(qLf, qR.data, aLf, aRf, p, nh);
+            for (int i = 0; i < NQ; ++i) fL[i] -= nc[NQ + i] * area;
+            for (int i = 0; i < NQ; ++i) fL[i] -= flux[i] * area;
+            cell_source(wb, qLf, X + (size_t)cL * NQ, p, nh, 1.0, area, fL);
+        } else if (wb) {
+            T qLf[NQ];
+            reconstruct(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
+                        m->cell_centroid + 3 * (size_t)cL, xf, vmin ? vmin + (size_t)cL * NQ : nullptr,
+                        vmax ? vmax + (size_t)cL * NQ : nullptr, use_limiters, qLf);
+            cell_source(wb, qLf, X + (size_t)cL * NQ, p, nh, 1.0, area, F + (size_t)cL * NQ);
+        }
+    }
+    for (int c = 0; c < m->ncells; ++c) {
+        T vol = m->cell_volume[c];
+        if (vol > 1e-15)
+            for (int i = 0; i < NQ; ++i) F[(size_t)c * NQ + i] /= vol;
+    }
+}
+
+struct Work {
+    std::vector<T> Xl, Al, G, vmin, vmax;
+};
+
+// TransportStep::FormRHS, TransportStep.hpp:325-357.  A_in stands for the scratch local aux
+// vector the reference passes (its contents feed only pass-through aux slots).
+void form_rhs(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T time, const T* X,
+              const T* A_in, T* F, Work& w) {
+    const size_t nc = m->ncells;
+    std::fill(F, F + nc * NQ, 0.0);
+    w.Xl.assign(X, X + nc * NQ);
+    w.Al.assign(A_in, A_in + nc * NA);
+    update_state(m->ncells, p, w.Xl.data(), w.Al.data());
+    const bool grad = o.order >= 2;
+    if (grad) {
+        w.vmin.resize(nc * NQ);
+        w.vmax.resize(nc * NQ);
+        neighbor_bounds(m, w.Xl.data(), w.vmin.data(), w.vmax.data());
+        w.G.resize(nc * NQ * DIM);
+        if (o.gradient == 1)
+            gradient_lsq(m, X, w.G.data());
+        else
+            gradient_gg(m, face_bc, p, X, w.G.data());
+    }
+    compute_fluxes(m, face_bc, p, time, w.Xl.data(), w.Al.data(), grad ? w.G.data() : nullptr,
+                   grad ? w.vmin.data() : nullptr, grad ? w.vmax.data() : nullptr, o.use_limiters,
+                   o.well_balanced, F);
+}
+
+// VirtualSolver::ComputeTimeStep, VirtualSolver.hpp:285-357
+T compute_time_step(const zdm_mesh* m, const T* p, const T* X, const T* A, T cfl, int* finite) {
+    T local_min = std::numeric_limits<T>::max();
+    for (int c = 0; c < m->ncells; ++c) {
+        const T* qc = X + (size_t)c * NQ;
+        const T* ac = A + (size_t)c * NA;
+        T eig = 0.0;
+        for (int d = 0; d < DIM; ++d) {
+            T n[3] = {0, 0, 0};
+            n[d] = 1.0;
+            auto r = NUM::local_max_abs_eigenvalue(qc, ac, p, n);
+            if (r[0] > eig) eig = r[0];
+        }
+        if (eig < 1e-14) continue;
+        const T* xc = m->cell_centroid + 3 * (size_t)c;
+        T rc = std::numeric_limits<T>::max();
+        for (int k = 0; k < m->cell_nfaces[c]; ++k) {
+            const T* xf = m->face_centroid + 3 * (size_t)m->cell_faces[(size_t)c * m->max_faces + k];
+            T d2 = 0.0;
+            for (int d = 0; d < DIM; ++d) {
+                T dd = xc[d] - xf[d];
+                d2 += dd * dd;
+            }
+            T dist = std::sqrt(d2);
+            if (dist < rc) rc = dist;
+        }
+        if (rc < std::numeric_limits<T>::max()) {
+            T ratio = rc / eig;
+            if (ratio < local_min) local_min = ratio;
+        }
+    }
+    if (!std::isfinite(local_min) || local_min >= 0.5 * std::numeric_limits<T>::max()) {
+        if (finite) *finite = 0;
+        return M::dt_max;
+    }
+    if (finite) *finite = 1;
+    return cfl * local_min;
+}
+
+// SourceStep::SolveLinearSystem, SourceStep.hpp:211-242 (GEPP)
+bool gepp(T* A, T* b, T* x, int n) {
+    for (int k = 0; k < n; ++k) {
+        int mr = k;
+        T mv = std::fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[i * n + k]) > mv) {
+                mv = std::fabs(A[i * n + k]);
+                mr = i;
+            }
+        if (mv < 1e-14) return false;
+        if (mr != k) {
+            for (int j = k; j < n; ++j) std::swap(A[k * n + j], A[mr * n + j]);
+            std::swap(b[k], b[mr]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            T fac = A[i * n + k] / A[k * n + k];
+            for (int j = k; j < n; ++j) A[i * n + j] -= fac * A[k * n + j];
+            b[i] -= fac * b[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        T sum = 0.0;
+        for (int j = i + 1; j < n; ++j) sum += A[i * n + j] * x[j];
+        x[i] = (b[i] - sum) / A[i * n + i];
+    }
+    return true;
+}
+
+// aux_jacobian_wrt_q (finite differences), SourceStep.hpp:26-53
+void aux_jac_fd(const T* q, const T* aux, const T* p, T* J) {
+    const T X0[3] = {0, 0, 0};
+    for (int i = 0; i < NA * NQ; ++i) J[i] = 0;
+    auto a0 = M::update_aux_variables(q, aux, p, 0.0, X0);
+    T qp[NQ];
+    for (int j = 0; j < NQ; ++j) {
+        for (int k = 0; k < NQ; ++k) qp[k] = q[k];
+        T h = (T)1e-7 * (std::fabs(q[j]) + (T)1e-7);
+        qp[j] += h;
+        auto a1 = M::update_aux_variables(qp, aux, p, 0.0, X0);
+        for (int k = 0; k < NA; ++k) J[k * NQ + j] = (a1[k] - a0[k]) / h;
+    }
+}
+
+// SourceStep::SolveLocalCell, SourceStep.hpp:125-209 (<= 15 Newton its, |R|_2 < 1e-8)
+int solve_local_cell(T dt, T* q, const T* aux_init, const T* p) {
+    const T X0[3] = {0, 0, 0};
+    T qo[NQ], qc[NQ], ac[NA], r[NQ], dl[NQ], J[NQ * NQ], Jx[NA * NQ];
+    for (int i = 0; i < NQ; ++i) qo[i] = qc[i] = q[i];
+    for (int i = 0; i < NA; ++i) ac[i] = aux_init ? aux_init[i] : 0.0;
+    int it = 0;
+    for (it = 0; it < 15; ++it) {
+        auto ua = M::update_aux_variables(qc, ac, p, 0.0, X0);
+        for (int i = 0; i < NA; ++i) ac[i] = ua[i];
+        auto S = M::source(qc, ac, p, (T)0, dt, X0);
+        T rn = 0.0;
+        for (int i = 0; i < NQ; ++i) {
+            r[i] = qc[i] - qo[i] - dt * S[i];
+            rn += r[i] * r[i];
+        }
+        if (std::sqrt(rn) < 1e-8) break;
+        auto dSdQ = M::source_jacobian_wrt_variables(qc, ac, p);
+        auto dSdA = M::source_jacobian_wrt_aux_variables(qc, ac, p);
+        aux_jac_fd(qc, ac, p, Jx);
+        for (int i = 0; i < NQ * NQ; ++i) J[i] = 0.0;
+        for (int i = 0; i < NQ; ++i) J[i * NQ + i] = 1.0;
+        for (int i = 0; i < NQ; ++i)
+            for (int j = 0; j < NQ; ++j) {
+                T term = dSdQ[i * NQ + j];
+                for (int k = 0; k < NA; ++k) term += dSdA[i * NA + k] * Jx[k * NQ + j];
+                J[i * NQ + j] -= dt * term;
+            }
+        for (int i = 0; i < NQ; ++i) r[i] = -r[i];
+        if (gepp(J, r, dl, NQ)) {
+            for (int i = 0; i < NQ; ++i) qc[i] += dl[i];
+        } else {
+            break;
+        }
+    }
+    for (int i = 0; i < NQ; ++i) q[i] = qc[i];
+    return it;
+}
+
+// SourceStep::Solve, SourceStep.hpp:72-122
+void source_solve(int nc, const T* p, T dt, T* X, const T* A) {
+    for (int c = 0; c < nc; ++c) solve_local_cell(dt, X + (size_t)c * NQ, A + (size_t)c * NA, p);
+}
+
+// MUSCLSolver::EnforcePhysicalConstraints, MUSCLSolver.hpp:234-254
+void enforce(int nc, const T* p, T* X, T* A) { update_state(nc, p, X, A); }
+
+// One TSStep: PETSc TSSSP "rks2" (s stages) for order >= 2, TSEULER otherwise
+// (SolverStrategies.hpp:26-38).  SSP-RK(s,2):
+//   u(i) = u(i-1) + dt/(s-1) L(u(i-1)),  i = 1..s-1          (VecAXPY)
+//   u+   = (s-1)/s u(s-1) + dt/s L(u(s-1)) + 1/s u           (VecAXPBYPCZ)
+// rhs(t, u, F) evaluates L; n = vector length.
+template <typename RHS>
+void ssp_or_euler(int order, int s, size_t n, T t0, T dt, T* X, RHS&& rhs) {
+    std::vector<T> F(n);
+    if (order < 2) {
+        rhs(t0, X, F.data());
+        for (size_t i = 0; i < n; ++i) X[i] += dt * F[i];
+        return;
+    }
+    std::vector<T> w0(X, X + n);
+    for (int i = 0; i < s - 1; ++i) {
+        T st = t0 + dt * (i / (s - 1.));
+        rhs(st, w0.data(), F.data());
+        const T a = dt / (s - 1.);
+        for (size_t k = 0; k < n; ++k) w0[k] += a * F[k];
+    }
+    rhs(t0 + dt, w0.data(), F.data());
+    const T al = (s - 1.) / s, be = dt / s, ga = 1. / s;
+    for (size_t k = 0; k < n; ++k) X[k] = al * w0[k] + be * F[k] + ga * X[k];
+}
+
+void ts_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T t0, T dt, T* X,
+             const T* A, Work& w) {
+    const size_t n = (size_t)m->ncells * NQ;
+    ssp_or_euler(o.order, o.ssp_stages, n, t0, dt, X,
+                 [&](T t, const T* U, T* F) { form_rhs(m, face_bc, p, o, t, U, A, F, w); });
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_model_info(int* nq, int* naux, int* npar, int* dim) {
+    *nq = NQ;
+    *naux = NA;
+    *npar = M::n_parameters;
+    *dim = DIM;
+    return 0;
+}
+
+int orc_default_params(double* p) {
+    auto v = M::default_parameters();
+    for (size_t i = 0; i < v.size(); ++i) p[i] = v[i];
+    return 0;
+}
+
+int orc_gradient(const zdm_mesh* m, const int* face_bc, const double* p, int method, const double* X,
+                 double* G) {
+    if (method == 1)
+        gradient_lsq(m, X, G);
+    else
+        gradient_gg(m, face_bc, p, X, G);
+    return 0;
+}
+
+int orc_bounds(const zdm_mesh* m, const double* X, double* vmin, double* vmax) {
+    neighbor_bounds(m, X, vmin, vmax);
+    return 0;
+}
+
+int orc_reconstruct(const double* q, const double* grad, const double* xc, const double* xf,
+                    const double* qmin, const double* qmax, int use_limiters, double* out) {
+    reconstruct(q, grad, xc, xf, qmin, qmax, use_limiters, out);
+    return 0;
+}
+
+int orc_form_rhs(const zdm_mesh* m, const int* face_bc, const double* p, int order, int use_limiters,
+                 int gradient, int well_balanced, double time, const double* X, const double* A,
+                 double* F) {
+    Opts o{order, 2, use_limiters, gradient, well_balanced, 0.5, 1.0, 1e-12};
+    Work w;
+    form_rhs(m, face_bc, p, o, time, X, A, F, w);
+    return 0;
+}
+
+int orc_flux(const double* qL, const double* qR, const double* aL, const double* aR, const double* p,
+             const double* n, double* flux, double* fluct) {
+    auto f = NUM::numerical_flux(qL, qR, aL, aR, p, n);
+    auto d = NUM::numerical_fluctuations(qL, qR, aL, aR, p, n);
+    for (int i = 0; i < NQ; ++i) flux[i] = f[i];
+    for (int i = 0; i < 2 * NQ; ++i) fluct[i] = d[i];
+    return 0;
+}
+
+int orc_eigen(const double* q, const double* a, const double* p, const double* n, double* lam) {
+    lam[0] = NUM::local_max_abs_eigenvalue(q, a, p, n)[0];
+    return 0;
+}
+
+int orc_update_state(int nc, const double* p, double* X, double* A) {
+    update_state(nc, p, X, A);
+    return 0;
+}
+
+double orc_compute_dt(const zdm_mesh* m, const double* p, const double* X, const double* A, double cfl,
+                      int* finite) {
+    return compute_time_step(m, p, X, A, cfl, finite);
+}
+
+int orc_source(int nc, const double* p, double dt, double* X, const double* A) {
+    source_solve(nc, p, dt, X, A);
+    return 0;
+}
+
+int orc_solve_cell(double dt, double* q, const double* aux, const double* p) {
+    return solve_local_cell(dt, q, aux, p);
+}
+
+int orc_gepp(double* A, double* b, double* x, int n) { return gepp(A, b, x, n) ? 0 : -1; }
+
+// The stage algebra on the scalar ODE u' = lam u (closed-form check of ssp_or_euler).
+double orc_ssp_linear(int order, int s, double lam, double t0, double dt, double u0) {
+    double u = u0;
+    ssp_or_euler(order, s, 1, t0, dt, &u, [&](double, const double* U, double* F) { F[0] = lam * U[0]; });
+    return u;
+}
+
+int orc_ts_step(const zdm_mesh* m, const int* face_bc, const double* p, int order, int ssp_stages,
+                int use_limiters, int gradient, int well_balanced, double t0, double dt, double* X,
+                const double* A) {
+    Opts o{order, ssp_stages, use_limiters, gradient, well_balanced, 0.5, 1.0, 1e-12};
+    Work w;
+    ts_step(m, face_bc, p, o, t0, dt, X, A, w);
+    return 0;
+}
+
+// MUSCLSolver::Run main loop, MUSCLSolver.hpp:54, 103, 140-188 (splitting, no MOOD):
+//   constraints; dt = max(CFL, min_dt); while t < t_end and step < max_steps:
+//     clamp dt to t_end; TSStep; constraints; source(dt); constraints; t += dt; dt = CFL
+// Returns the number of steps taken; *time and *dt_next are updated.
+int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, int ssp_stages,
+            int use_limiters, int gradient, int well_balanced, double cfl, double t_end,
+            double min_dt, int max_steps, double* X, double* A, double* time, double* dt_io,
+            int first) {
+    Opts o{order, ssp_stages, use_limiters, gradient, well_balanced, cfl, t_end, min_dt};
+    Work w;
+    double t = *time;
+    double dt = *dt_io;
+    if (first) {
+        enforce(m->ncells, p, X, A);
+        dt = std::max(compute_time_step(m, p, X, A, cfl, nullptr), min_dt);
+    }
+    int steps = 0;
+    while (t < t_end && steps < max_steps) {
+        double t0 = t, dt0 = dt;
+        if (t0 + dt0 > t_end) dt0 = t_end - t0;
+        ts_step(m, face_bc, p, o, t0, dt0, X, A, w);
+        enforce(m->ncells, p, X, A);
+        source_solve(m->ncells, p, dt0, X, A);
+        enforce(m->ncells, p, X, A);
+        ++steps;
+        t = t0 + dt0;
+        dt = compute_time_step(m, p, X, A, cfl, nullptr);
+    }
+    *time = t;
+    *dt_io = dt;
+    return steps;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+import ctypes
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = os.path.join(ROOT, "zoomy-dmplex_amd")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+
+def load_package():
+    if "zoomy_dmplex_amd" in sys.modules:
+        return sys.modules["zoomy_dmplex_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "zoomy_dmplex_amd", os.path.join(PKG_DIR, "__init__.py"),
+        submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["zoomy_dmplex_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+zdm = load_package()
+import zoomy_dmplex_amd.cases as cases  # noqa: E402
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def dp(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def ip(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Oracle:
+    """ctypes view of oracle/liboracle_<model>.so (the CPU checker)."""
+
+    def __init__(self, model):
+        path = os.path.join(ORACLE_DIR, "liboracle_%s.so" % model)
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: %s (run oracle/build_oracle.py)" % path)
+        L = ctypes.CDLL(path)
+        M = ctypes.POINTER(zdm._ZdmMesh)
+        L.orc_model_info.argtypes = [_ip] * 4
+        L.orc_default_params.argtypes = [_dp]
+        L.orc_gradient.argtypes = [M, _ip, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_bounds.argtypes = [M, _dp, _dp, _dp]
+        L.orc_reconstruct.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]
+        L.orc_form_rhs.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_flux.argtypes = [_dp] * 8
+        L.orc_eigen.argtypes = [_dp] * 5
+        L.orc_update_state.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_compute_dt.argtypes = [M, _dp, _dp, _dp, ctypes.c_double, _ip]
+        L.orc_compute_dt.restype = ctypes.c_double
+        L.orc_source.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
+        L.orc_solve_cell.argtypes = [ctypes.c_double, _dp, _dp, _dp]
+        L.orc_gepp.argtypes = [_dp, _dp, _dp, ctypes.c_int]
+        L.orc_ssp_linear.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4
+        L.orc_ssp_linear.restype = ctypes.c_double
+        L.orc_ts_step.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  _dp, _dp]
+        L.orc_run.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+        self.L = L
+        v = [ctypes.c_int() for _ in range(4)]
+        L.orc_model_info(*[ctypes.byref(x) for x in v])
+        self.n_q, self.n_aux, self.n_params, self.dim = [x.value for x in v]
+        p = (ctypes.c_double * self.n_params)()
+        L.orc_default_params(p)
+        self.default_params = np.array(p[:])
+
+    def flux(self, qL, qR, aL, aR, p, n):
+        f = np.zeros(self.n_q)
+        d = np.zeros(2 * self.n_q)
+        args = [np.ascontiguousarray(x, dtype=np.float64) for x in (qL, qR, aL, aR, p, n)]
+        self.L.orc_flux(*[dp(a) for a in args], dp(f), dp(d))
+        return f, d
+
+    def eigen(self, q, a, p, n):
+        out = np.zeros(1)
+        args = [np.ascontiguousarray(x, dtype=np.float64) for x in (q, a, p, n)]
+        self.L.orc_eigen(*[dp(x) for x in args], dp(out))
+        return out[0]
+
+    def form_rhs(self, mesh, face_bc, p, X, A, order=2, use_limiters=1, gradient=0, t=0.0,
+                 well_balanced=0):
+        F = np.zeros_like(X)
+        self.L.orc_form_rhs(mesh.handle, ip(face_bc), dp(p), order, use_limiters, gradient,
+                            int(well_balanced), t, dp(X), dp(A), dp(F))
+        return F
+
+    def gradient(self, mesh, face_bc, p, X, method=0):
+        G = np.zeros((mesh.ncells, self.n_q, self.dim))
+        self.L.orc_gradient(mesh.handle, ip(face_bc), dp(p), method, dp(X), dp(G))
+        return G
+
+    def bounds(self, mesh, X):
+        lo, hi = np.zeros_like(X), np.zeros_like(X)
+        self.L.orc_bounds(mesh.handle, dp(X), dp(lo), dp(hi))
+        return lo, hi
+
+    def update_state(self, p, X, A):
+        self.L.orc_update_state(X.shape[0], dp(p), dp(X), dp(A))
+
+    def source(self, p, dt, X, A):
+        self.L.orc_source(X.shape[0], dp(p), float(dt), dp(X), dp(A))
+
+    def compute_dt(self, mesh, p, X, A, cfl):
+        fin = ctypes.c_int()
+        return self.L.orc_compute_dt(mesh.handle, dp(p), dp(X), dp(A), cfl, ctypes.byref(fin))
+
+    def run(self, mesh, face_bc, p, X, A, order=2, ssp_stages=2, use_limiters=1, gradient=0,
+            cfl=0.45, t_end=1e9, min_dt=1e-12, max_steps=10, time=0.0, dt=0.0, first=True,
+            well_balanced=0):
+        t = np.array([time])
+        d = np.array([dt])
+        n = self.L.orc_run(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters,
+                           gradient, int(well_balanced), cfl, t_end, min_dt, max_steps, dp(X),
+                           dp(A), dp(t), dp(d), int(first))
+        return n, t[0], d[0]
+
+
+_oracles = {}
+
+
+def oracle(model):
+    if model not in _oracles:
+        _oracles[model] = Oracle(model)
+    return _oracles[model]
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: test needs an MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def pkg():
+    return zdm
+
+
+@pytest.fixture(scope="session")
+def case_mod():
+    return cases

@@ -1,0 +1,437 @@
+"""zoomy-dmplex_amd: MI355X-native finite-volume time step for SME / VAM shallow-flow models.
+
+Python host layer over the C ABI (include/zdm.h, include/zdm_mesh.h).  It mirrors the
+reference's host objects:
+
+  * Settings            -- Settings.hpp:13-138 (JSON keys and defaults)
+  * Mesh                -- the DMPlex services the step uses (geometry, adjacency, tags)
+  * boundary_face_bc    -- VirtualSolver.hpp:394-417, 500-511 (tag -> model BC index)
+  * Solver              -- MUSCLSolver::Run (MUSCLSolver.hpp:22-194) on the device
+
+The compute path is the HIP library libzdm_<model>.so built in-tree (see build.py); there
+is no fallback: a missing library raises.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+class ZdmError(RuntimeError):
+    pass
+
+
+def _load(name):
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        raise ZdmError("%s is not built (run zoomy-dmplex_amd/build.py or __graft_entry__.build())"
+                       % path)
+    return ctypes.CDLL(path)
+
+
+# --------------------------------------------------------------------------- mesh service
+
+class _ZdmMesh(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("ncells", ctypes.c_int), ("nfaces", ctypes.c_int),
+        ("max_faces", ctypes.c_int), ("nverts", ctypes.c_int), ("max_cell_verts", ctypes.c_int),
+        ("cell_centroid", _c_dbl_p), ("cell_volume", _c_dbl_p), ("cell_nfaces", _c_int_p),
+        ("cell_faces", _c_int_p), ("cell_nbrs", _c_int_p), ("cell_perm", _c_int_p),
+        ("cell_verts", _c_int_p), ("face_cells", _c_int_p), ("face_normal", _c_dbl_p),
+        ("face_centroid", _c_dbl_p), ("face_tag", _c_int_p), ("verts", _c_dbl_p),
+        ("nphysical", ctypes.c_int), ("physical_tag", _c_int_p), ("physical_dim", _c_int_p),
+        ("physical_names", ctypes.c_char_p),
+    ]
+
+
+_mesh_lib = None
+
+
+def mesh_lib():
+    global _mesh_lib
+    if _mesh_lib is None:
+        lib = _load("libzdm_mesh.so")
+        P = ctypes.POINTER(ctypes.POINTER(_ZdmMesh))
+        lib.zdm_mesh_tri_grid.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_uint64, ctypes.c_int, P]
+        lib.zdm_mesh_line.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, P]
+        lib.zdm_mesh_read_msh.argtypes = [ctypes.c_char_p, ctypes.c_int, P]
+        lib.zdm_mesh_from_cells.argtypes = [ctypes.c_int, ctypes.c_int, _c_dbl_p, ctypes.c_int,
+                                            ctypes.c_int, _c_int_p, ctypes.c_int, _c_int_p,
+                                            _c_int_p, ctypes.c_int, ctypes.c_int, P]
+        lib.zdm_mesh_destroy.argtypes = [ctypes.POINTER(_ZdmMesh)]
+        lib.zdm_mesh_last_error.restype = ctypes.c_char_p
+        lib.zdm_splitmix64_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        lib.zdm_splitmix64_next.restype = ctypes.c_uint64
+        _mesh_lib = lib
+    return _mesh_lib
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Mesh:
+    """Finite-volume mesh in solver order (cells RCM-renumbered, faces by support)."""
+
+    def __init__(self, ptr):
+        self._ptr = ptr
+        m = ptr.contents
+        self.dim, self.ncells, self.nfaces = m.dim, m.ncells, m.nfaces
+        self.max_faces = m.max_faces
+        nc, nf, mf = self.ncells, self.nfaces, self.max_faces
+        self.cell_centroid = _arr(m.cell_centroid, nc * 3, np.float64).reshape(nc, 3)
+        self.cell_volume = _arr(m.cell_volume, nc, np.float64)
+        self.cell_nfaces = _arr(m.cell_nfaces, nc, np.int32)
+        self.cell_faces = _arr(m.cell_faces, nc * mf, np.int32).reshape(nc, mf)
+        self.cell_nbrs = _arr(m.cell_nbrs, nc * mf, np.int32).reshape(nc, mf)
+        self.cell_perm = _arr(m.cell_perm, nc, np.int32)
+        self.face_cells = _arr(m.face_cells, nf * 2, np.int32).reshape(nf, 2)
+        self.face_normal = _arr(m.face_normal, nf * 3, np.float64).reshape(nf, 3)
+        self.face_centroid = _arr(m.face_centroid, nf * 3, np.float64).reshape(nf, 3)
+        self.face_tag = _arr(m.face_tag, nf, np.int32)
+        self.physical = []
+        if m.nphysical > 0:
+            raw = ctypes.string_at(m.physical_names, 64 * m.nphysical)
+            tags = _arr(m.physical_tag, m.nphysical, np.int32)
+            dims = _arr(m.physical_dim, m.nphysical, np.int32)
+            for i in range(m.nphysical):
+                name = raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
+                self.physical.append((name, int(dims[i]), int(tags[i])))
+
+    @property
+    def handle(self):
+        return self._ptr
+
+    def __del__(self):
+        try:
+            if self._ptr:
+                mesh_lib().zdm_mesh_destroy(self._ptr)
+                self._ptr = None
+        except Exception:
+            pass
+
+    @staticmethod
+    def _make(fn, *args):
+        out = ctypes.POINTER(_ZdmMesh)()
+        rc = fn(*args, ctypes.byref(out))
+        if rc != 0:
+            raise ZdmError(mesh_lib().zdm_mesh_last_error().decode())
+        return Mesh(out)
+
+    @classmethod
+    def tri_grid(cls, nx, ny=None, jitter=0.0, seed=0, renumber=True):
+        """Unit square, nx x ny squares split SW->NE into right triangles; walls (tag 1)."""
+        return cls._make(mesh_lib().zdm_mesh_tri_grid, nx, ny or nx, float(jitter),
+                         ctypes.c_uint64(seed), int(renumber))
+
+    @classmethod
+    def line(cls, n, x0=0.0, x1=1.0, renumber=False):
+        return cls._make(mesh_lib().zdm_mesh_line, n, float(x0), float(x1), int(renumber))
+
+    @classmethod
+    def read_msh(cls, path, renumber=True):
+        """Gmsh MSH 2.2 (the reference's mesh format, VirtualSolver.hpp:388)."""
+        return cls._make(mesh_lib().zdm_mesh_read_msh, path.encode(), int(renumber))
+
+    @classmethod
+    def from_cells(cls, dim, xyz, cells, bfaces=None, btags=None, mark_all_tag=1, renumber=True):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        nbf = 0 if bfaces is None else len(btags)
+        bf = np.ascontiguousarray(bfaces if bfaces is not None else np.zeros(1), dtype=np.int32)
+        bt = np.ascontiguousarray(btags if btags is not None else np.zeros(1), dtype=np.int32)
+        return cls._make(mesh_lib().zdm_mesh_from_cells, dim, xyz.shape[0],
+                         xyz.ctypes.data_as(_c_dbl_p), cells.shape[0], cells.shape[1],
+                         cells.ctypes.data_as(_c_int_p), nbf, bf.ctypes.data_as(_c_int_p),
+                         bt.ctypes.data_as(_c_int_p), mark_all_tag, int(renumber))
+
+
+def boundary_face_bc(mesh, model_tags, model_dim=None):
+    """Model BC index per face (-1 = interior or no model BC for its tag).
+
+    VirtualSolver.hpp:394-417: every model tag except "default" must be a physical name of
+    the boundary dimension, else the run is refused.  VirtualSolver.hpp:500-511: a tag name
+    found in the mesh maps its physical tag; "default" otherwise maps tag 1.
+    """
+    model_dim = mesh.dim if model_dim is None else model_dim
+    mesh_map = {name: (d, t) for (name, d, t) in mesh.physical}
+    for name in model_tags:
+        if name == "default":
+            continue
+        if name not in mesh_map or mesh_map[name][0] != model_dim - 1:
+            raise ZdmError("GMSH boundaries must exactly match model tags (excluding 'default').")
+    bmap = {}
+    for i, name in enumerate(model_tags):
+        tag = -1
+        if name in mesh_map:
+            tag = mesh_map[name][1]
+        elif name == "default":
+            tag = 1
+        if tag != -1:
+            bmap[tag] = i
+    out = np.full(mesh.nfaces, -1, dtype=np.int32)
+    bnd = mesh.face_cells[:, 1] < 0
+    for f in np.nonzero(bnd)[0]:
+        out[f] = bmap.get(int(mesh.face_tag[f]), -1)
+    return out
+
+
+# ------------------------------------------------------------------------------- settings
+
+class Settings:
+    """JSON settings with the reference's keys and defaults (Settings.hpp:13-138).  Unknown
+    keys are ignored; extension keys: solver.gradient ("green_gauss" | "lsq"),
+    solver.well_balanced (bool)."""
+
+    SOLVER_DEFAULTS = dict(t_end=1.0, cfl=0.5, use_deep_adjacency=True, reconstruction_order=1,
+                           min_dt=1.0e-12, limiter="venkatakrishnan", time_integration="splitting",
+                           method="muscl", refresh_derivative_aux=True, positivity="none",
+                           wet_dry_eps=1.0e-2, ssp_stages=5, gradient="green_gauss",
+                           well_balanced=False)
+    IO_DEFAULTS = dict(directory="output", filename="sol", snapshots=10, snapshot_logic="snap",
+                       clean_directory=False, mesh_path="", mesh_label="Face Sets",
+                       initial_condition_file="", initial_condition_mask=[], write_3d=False,
+                       n_layers_3d=10, output_3d_name="sol_3d")
+
+    def __init__(self, d=None):
+        d = d or {}
+        self.name = d.get("name", "Simulation")
+        self.io = dict(self.IO_DEFAULTS)
+        self.io.update({k: v for k, v in d.get("io", {}).items() if k in self.IO_DEFAULTS})
+        self.io["initial_condition_mask"] = [int(v) for v in self.io["initial_condition_mask"]
+                                             if isinstance(v, int)]
+        self.solver = dict(self.SOLVER_DEFAULTS)
+        self.solver.update({k: v for k, v in d.get("solver", {}).items()
+                            if k in self.SOLVER_DEFAULTS})
+        params = d.get("model", {}).get("parameters", {})
+        self.parameters = {k: float(v) for k, v in params.items()
+                           if isinstance(v, (int, float)) and not isinstance(v, bool)}
+
+    @classmethod
+    def from_json(cls, path):
+        if not os.path.exists(path):
+            raise ZdmError("Could not open settings file: %s" % path)
+        with open(path) as f:
+            return cls(json.load(f))
+
+
+# --------------------------------------------------------------------------------- solver
+
+class _Options(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int), ("ssp_stages", ctypes.c_int),
+                ("use_limiters", ctypes.c_int), ("gradient", ctypes.c_int),
+                ("source", ctypes.c_int), ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("min_dt", ctypes.c_double), ("block_size", ctypes.c_int),
+                ("well_balanced", ctypes.c_int)]
+
+
+_model_libs = {}
+
+SOLVER_SYMBOLS = [
+    "zdm_model_info", "zdm_model_name", "zdm_model_default_params", "zdm_model_boundary_tags",
+    "zdm_model_parameter_names",
+    "zdm_solver_create", "zdm_solver_destroy", "zdm_state_upload", "zdm_state_download",
+    "zdm_initialize", "zdm_step", "zdm_run", "zdm_get_time", "zdm_synchronize",
+    "zdm_transport_rhs", "zdm_source_step", "zdm_enforce_constraints", "zdm_cfl_dt",
+    "zdm_state_upload_device", "zdm_state_download_device", "zdm_enable_timing",
+    "zdm_kernel_time", "zdm_last_error",
+]
+
+
+def model_lib(model):
+    if model in _model_libs:
+        return _model_libs[model]
+    lib = _load("libzdm_%s.so" % model)
+    V = ctypes.c_void_p
+    lib.zdm_model_info.argtypes = [_c_int_p] * 4
+    lib.zdm_model_name.restype = ctypes.c_char_p
+    lib.zdm_model_default_params.argtypes = [_c_dbl_p]
+    lib.zdm_model_boundary_tags.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    lib.zdm_model_parameter_names.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    lib.zdm_solver_create.argtypes = [ctypes.POINTER(_ZdmMesh), _c_int_p, _c_dbl_p,
+                                      ctypes.POINTER(_Options), ctypes.POINTER(V)]
+    lib.zdm_solver_destroy.argtypes = [V]
+    lib.zdm_state_upload.argtypes = [V, _c_dbl_p, _c_dbl_p]
+    lib.zdm_state_download.argtypes = [V, _c_dbl_p, _c_dbl_p]
+    lib.zdm_state_upload_device.argtypes = [V, V, V]
+    lib.zdm_state_download_device.argtypes = [V, V, V]
+    lib.zdm_initialize.argtypes = [V]
+    lib.zdm_step.argtypes = [V, ctypes.c_int]
+    lib.zdm_run.argtypes = [V, ctypes.c_int, ctypes.c_int, _c_int_p]
+    lib.zdm_get_time.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_int_p]
+    lib.zdm_synchronize.argtypes = [V]
+    lib.zdm_transport_rhs.argtypes = [V, ctypes.c_double, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+    lib.zdm_source_step.argtypes = [V, ctypes.c_double]
+    lib.zdm_enforce_constraints.argtypes = [V]
+    lib.zdm_cfl_dt.argtypes = [V, _c_dbl_p]
+    lib.zdm_enable_timing.argtypes = [V, ctypes.c_int]
+    lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
+    lib.zdm_last_error.restype = ctypes.c_char_p
+    _model_libs[model] = lib
+    return lib
+
+
+class ModelInfo:
+    def __init__(self, model):
+        lib = model_lib(model)
+        v = [ctypes.c_int() for _ in range(4)]
+        lib.zdm_model_info(*[ctypes.byref(x) for x in v])
+        self.name = model
+        self.n_q, self.n_aux, self.n_params, self.dim = [x.value for x in v]
+        p = (ctypes.c_double * max(self.n_params, 1))()
+        lib.zdm_model_default_params(p)
+        self.default_params = np.array(p[:self.n_params])
+        buf = ctypes.create_string_buffer(64 * 16)
+        n = lib.zdm_model_boundary_tags(buf, 16)
+        self.boundary_tags = [buf.raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
+                              for i in range(n)]
+        buf = ctypes.create_string_buffer(64 * 16)
+        n = lib.zdm_model_parameter_names(buf, 16)
+        self.parameter_names = [buf.raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
+                                for i in range(n)]
+
+
+def _dp(a):
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+class Solver:
+    """MUSCL / SSP-RK finite-volume solver on one GPU (MUSCLSolver, MUSCLSolver.hpp:14-194).
+
+    Construction mirrors VirtualSolver::Initialize + InitializeComponents: parameters start
+    from Model::default_parameters() and are overridden by name (VirtualSolver.hpp:199-217).
+    """
+
+    def __init__(self, model, mesh, settings=None, params=None, face_bc=None, source=True,
+                 order=None, ssp_stages=None, cfl=None, t_end=None, gradient=None,
+                 use_limiters=None, well_balanced=None):
+        self.model = model
+        self.info = ModelInfo(model)
+        self.lib = model_lib(model)
+        self.mesh = mesh
+        st = settings if settings is not None else Settings()
+        sv = st.solver
+        p = self.info.default_params.copy()
+        names = []
+        buf = params if params is not None else {}
+        if isinstance(buf, dict):
+            merged = dict(st.parameters)
+            merged.update(buf)
+            pn = self.info.parameter_names
+            for k, v in merged.items():
+                if k in pn:
+                    p[pn.index(k)] = v
+                else:
+                    names.append(k)
+        else:
+            p = np.asarray(buf, dtype=np.float64)
+        self.params = p
+        self.unknown_params = names
+        if face_bc is None:
+            face_bc = boundary_face_bc(mesh, self.info.boundary_tags, self.info.dim)
+        self.face_bc = np.ascontiguousarray(face_bc, dtype=np.int32)
+        o = _Options()
+        o.order = int(order if order is not None else sv["reconstruction_order"])
+        o.ssp_stages = int(ssp_stages if ssp_stages is not None else sv["ssp_stages"])
+        o.use_limiters = int(use_limiters if use_limiters is not None else sv["limiter"] != "none")
+        g = gradient if gradient is not None else sv.get("gradient", "green_gauss")
+        o.gradient = 1 if g in ("lsq", "least_squares", 1) else 0
+        o.source = int(source and sv["time_integration"] == "splitting")
+        o.cfl = float(cfl if cfl is not None else sv["cfl"])
+        o.t_end = float(t_end if t_end is not None else sv["t_end"])
+        o.min_dt = float(sv["min_dt"])
+        o.block_size = 0
+        o.well_balanced = int(bool(well_balanced if well_balanced is not None
+                                   else sv.get("well_balanced", False)))
+        self.options = o
+        h = ctypes.c_void_p()
+        rc = self.lib.zdm_solver_create(mesh.handle, self.face_bc.ctypes.data_as(_c_int_p),
+                                        _dp(self.params), ctypes.byref(o), ctypes.byref(h))
+        self._check(rc)
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise ZdmError(self.lib.zdm_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.zdm_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, X, A=None):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(self.mesh.ncells, self.info.n_q)
+        if A is not None:
+            A = np.ascontiguousarray(A, dtype=np.float64).reshape(self.mesh.ncells, self.info.n_aux)
+        self._check(self.lib.zdm_state_upload(self._h, _dp(X), _dp(A) if A is not None else None))
+
+    def download(self):
+        X = np.empty((self.mesh.ncells, self.info.n_q))
+        A = np.empty((self.mesh.ncells, self.info.n_aux))
+        self._check(self.lib.zdm_state_download(self._h, _dp(X), _dp(A)))
+        return X, A
+
+    def initialize(self):
+        self._check(self.lib.zdm_initialize(self._h))
+
+    def step(self, n=1):
+        self._check(self.lib.zdm_step(self._h, int(n)))
+
+    def run(self, max_steps=1 << 30, check_every=16):
+        n = ctypes.c_int()
+        self._check(self.lib.zdm_run(self._h, int(max_steps), int(check_every), ctypes.byref(n)))
+        return n.value
+
+    def synchronize(self):
+        self._check(self.lib.zdm_synchronize(self._h))
+
+    def time(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._check(self.lib.zdm_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
+        return t.value, dt.value, n.value
+
+    def transport_rhs(self, X, A=None, t=0.0):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        F = np.empty_like(X)
+        Ap = None
+        if A is not None:
+            A = np.ascontiguousarray(A, dtype=np.float64)
+            Ap = _dp(A)
+        self._check(self.lib.zdm_transport_rhs(self._h, float(t), _dp(X), Ap, _dp(F)))
+        return F
+
+    def source_step(self, dt):
+        self._check(self.lib.zdm_source_step(self._h, float(dt)))
+
+    def enforce_constraints(self):
+        self._check(self.lib.zdm_enforce_constraints(self._h))
+
+    def cfl_dt(self):
+        d = ctypes.c_double()
+        self._check(self.lib.zdm_cfl_dt(self._h, ctypes.byref(d)))
+        return d.value
+
+    def enable_timing(self, on=True):
+        self._check(self.lib.zdm_enable_timing(self._h, int(on)))
+
+    def kernel_time(self, kernel):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        self._check(self.lib.zdm_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+

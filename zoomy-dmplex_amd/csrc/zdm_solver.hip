@@ -1,0 +1,1271 @@
+// MI355X (gfx950) finite-volume time step for generated SME / VAM models.
+//
+// Built once per model directory (Model.H + Numerics.H on the include path), as the
+// reference builds solver_cpu on its generated headers.  The headers are consumed unchanged:
+// they are compiled for host and device through clang's force_cuda_host_device pragma.
+//
+// Device design (one GPU; FP64 throughout; no atomics; results independent of scheduling):
+//   * structure-of-arrays state x[i*ld + c], cells renumbered by reverse Cuthill-McKee;
+//   * k_prep  (per cell): wet/dry clamp + aux, face-neighbour bounds, Green-Gauss or
+//     least-squares gradient, limited MUSCL reconstruction of every face of the cell into a
+//     per-cell face-value buffer (so the gradient and bounds never go back to HBM);
+//   * k_flux  (per cell): numerical flux + fluctuations of each face of the cell, gathered in
+//     ascending face order (the reference's scatter order, TransportStep.hpp:403-466), divided
+//     by the volume, with the SSP-RK stage update fused into the epilogue;
+//   * k_post  (per cell): constraints -> implicit source Newton -> constraints -> local CFL
+//     ratio, reduced per workgroup; k_finalize turns the partial minima into the next dt and
+//     keeps time / dt on the device, so a time step needs no host synchronisation.
+// Workgroups are remapped so that each XCD (own L2) works on a contiguous range of cells.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "zdm.h"
+
+#pragma clang force_cuda_host_device begin
+#include "Numerics.H"
+#pragma clang force_cuda_host_device end
+
+#ifndef ZDM_MODEL_NAME
+#define ZDM_MODEL_NAME "model"
+#endif
+
+// Optional model hook (not emitted by the reference generator): centred bathymetry term of
+// the second-order hydrostatic reconstruction, added per face side when well_balanced is on.
+template <typename TT, typename = void>
+struct HasCellSource : std::false_type {};
+template <typename TT>
+struct HasCellSource<TT, std::void_t<decltype(Model<TT>::hydrostatic_cell_source(
+                             std::declval<const TT*>(), std::declval<const TT*>(),
+                             std::declval<const TT*>(), std::declval<const TT*>()))>>
+    : std::true_type {};
+
+namespace {
+
+using T = double;
+using M = Model<T>;
+using NUM = Numerics<T>;
+constexpr int NQ = M::n_dof_q;
+constexpr int NA = M::n_dof_qaux;
+constexpr int DIM = M::dimension;
+constexpr int NPAR = M::n_parameters;
+constexpr int MAXPAR = 16;
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;
+static_assert(NPAR <= MAXPAR, "too many model parameters");
+
+thread_local std::string g_err;
+int fail(const std::string& m) {
+    g_err = m;
+    return -1;
+}
+#define HIPCHK(x)                                                                     \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) return fail(std::string(#x " : ") + hipGetErrorString(e_)); \
+    } while (0)
+
+struct Params {
+    T p[MAXPAR];
+};
+
+// Device control block: time stepping state lives on the device between steps.
+struct Ctrl {
+    T t;          // simulation time at the start of the current step
+    T dt;         // dt of the current step (clamped to t_end)
+    T dt_cfl;     // unclamped CFL dt (ComputeTimeStep) for the current step
+    T dt_query;   // result slot of a standalone ComputeTimeStep
+    T a_stage;    // dt / (s-1)          (Euler: dt)
+    T al, be, ga; // (s-1)/s, dt/s, 1/s
+    T stage_t[16];
+    int active;   // t < t_end
+    int steps;
+    int finite;
+    int pad;
+};
+
+struct DevMesh {
+    int ncells, nfaces, ld;
+    const T* vol;
+    const T* cx;
+    const T* cy;
+    const T* rc;      // min centroid -> face-centroid distance
+    const int* cface; // [MAXF][ld]
+    const int* cnbr;  // [MAXF][ld]
+    const int* cmeta; // [MAXF][ld]: bits 0-1 = slot of the face in the neighbour, bit 2 = this
+                      // cell is support[1]
+    const T* fNx;     // area-weighted normal
+    const T* fNy;
+    const T* fcx;     // face centroid
+    const T* fcy;
+    const T* farea;   // sqrt(N.N)
+    const T* fnhx;    // N / area
+    const T* fnhy;
+    const int* fbc;   // model BC index of boundary faces, -1 = none
+    const T* lsqw;    // [MAXF][DIM][ld] least-squares weights
+};
+
+__device__ __forceinline__ int cell_index() {
+    // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give each XCD a
+    // contiguous range of workgroups so face neighbours share that XCD's L2.
+    const int nb = gridDim.x;
+    const int xcd = blockIdx.x % NXCD, idx = blockIdx.x / NXCD;
+    const int q = nb / NXCD, r = nb % NXCD;
+    const int lb = xcd * q + (xcd < r ? xcd : r) + idx;
+    return lb * blockDim.x + threadIdx.x;
+}
+
+template <int N>
+__device__ __forceinline__ void load_soa(const T* __restrict__ a, int ld, int c, T* v) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = a[(size_t)i * ld + c];
+}
+template <int N>
+__device__ __forceinline__ void store_soa(T* __restrict__ a, int ld, int c, const T* v) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[(size_t)i * ld + c] = v[i];
+}
+
+// TransportStep::UpdateState per cell (TransportStep.hpp:210-227)
+__device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
+    const T X0[3] = {0, 0, 0};
+    auto rq = M::update_variables(q, a, P.p, 0.0);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) q[i] = rq[i];
+    auto ra = M::update_aux_variables(q, a, P.p, 0.0, X0);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) a[i] = ra[i];
+}
+
+// ------------------------------------------------------------------------------ k_prep
+// GRAD: 0 = none (PCM), 1 = Green-Gauss, 2 = least squares
+template <int MAXF, int GRAD>
+__global__ __launch_bounds__(BLOCK) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
+                                                const T* __restrict__ X, const T* __restrict__ Ag,
+                                                T* __restrict__ As, T* __restrict__ QF, int use_limiters) {
+    if (!ctrl->active) return;
+    const int c = cell_index();
+    if (c >= m.ncells) return;
+    const int ld = m.ld;
+    T q[NQ], qc[NQ], ac[NA];
+    load_soa<NQ>(X, ld, c, q);
+    load_soa<NA>(Ag, ld, c, ac);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) qc[i] = q[i];
+    update_state_cell(P, qc, ac);
+    store_soa<NA>(As, ld, c, ac);
+
+    if constexpr (GRAD == 0) {
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            if (m.cface[(size_t)k * ld + c] < 0) continue;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) QF[((size_t)k * NQ + i) * ld + c] = qc[i];
+        }
+        return;
+    } else {
+        T lo[NQ], hi[NQ], g[NQ * DIM];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            lo[i] = qc[i];
+            hi[i] = qc[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NQ * DIM; ++i) g[i] = 0;
+        const T xc[2] = {m.cx[c], DIM > 1 ? m.cy[c] : (T)0};
+
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            const int f = m.cface[(size_t)k * ld + c];
+            if (f < 0) continue;
+            const int nb = m.cnbr[(size_t)k * ld + c];
+            const T N[2] = {m.fNx[f], DIM > 1 ? m.fNy[f] : (T)0};
+            if (nb >= 0) {
+                T qn[NQ], an[NA];
+                load_soa<NQ>(X, ld, nb, qn);
+                load_soa<NA>(Ag, ld, nb, an);
+                auto rn = M::update_variables(qn, an, P.p, 0.0);
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    if (rn[i] < lo[i]) lo[i] = rn[i];
+                    if (rn[i] > hi[i]) hi[i] = rn[i];
+                }
+                if constexpr (GRAD == 1) {
+                    const bool right = (m.cmeta[(size_t)k * ld + c] >> 2) & 1;
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i) {
+                        const T fv = (T)0.5 * (q[i] + qn[i]);
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            const T v = fv * N[d];
+                            if (right)
+                                g[i * DIM + d] -= v;
+                            else
+                                g[i * DIM + d] += v;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i) {
+                        const T du = qn[i] - q[i];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d)
+                            g[i * DIM + d] += m.lsqw[((size_t)k * DIM + d) * ld + c] * du;
+                    }
+                }
+            } else if constexpr (GRAD == 1) {
+                const int bc = m.fbc[f];
+                if (bc >= 0) {
+                    const T nh[3] = {m.fnhx[f], DIM > 1 ? m.fnhy[f] : (T)0, 0};
+                    const T xf[3] = {m.fcx[f], DIM > 1 ? m.fcy[f] : (T)0, 0};
+                    auto qb = M::boundary_conditions(bc, q, nullptr, P.p, nh, xf, 0.0, 0.0);
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i)
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) g[i * DIM + d] += qb[i] * N[d];
+                }
+            }
+        }
+        if constexpr (GRAD == 1) {
+            const T vol = m.vol[c];
+            if (vol > 1e-15) {
+#pragma unroll
+                for (int i = 0; i < NQ * DIM; ++i) g[i] /= vol;
+            }
+        }
+        // limited linear reconstruction at every face of the cell (Reconstruction.hpp:203-243)
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            const int f = m.cface[(size_t)k * ld + c];
+            if (f < 0) continue;
+            const T dx[2] = {m.fcx[f] - xc[0], DIM > 1 ? m.fcy[f] - xc[1] : (T)0};
+            T delta[NQ], fin[NQ];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                delta[i] = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) delta[i] += g[i * DIM + d] * dx[d];
+                fin[i] = 1.0;
+            }
+            if (use_limiters) {
+                // vacuum limiter: first order next to (nearly) dry cells
+                if (NQ > 1 && lo[1] < 1e-2) {
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i) fin[i] = 0.0;
+                }
+                // TVD / Barth-Jespersen, component-wise
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    const T unl = qc[i] + delta[i];
+                    const T dq = unl - qc[i];
+                    if (std::fabs(dq) < 1e-12) continue;
+                    T a = 1.0;
+                    if (dq > 0) {
+                        if (hi[i] > qc[i])
+                            a = std::min((T)1.0, (hi[i] - qc[i]) / dq);
+                        else
+                            a = 0.0;
+                    } else {
+                        if (lo[i] < qc[i])
+                            a = std::min((T)1.0, (lo[i] - qc[i]) / dq);
+                        else
+                            a = 0.0;
+                    }
+                    if (a < fin[i]) fin[i] = a;
+                }
+            }
+            T alpha = 1.;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                if (alpha > fin[i]) alpha = fin[i];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) QF[((size_t)k * NQ + i) * ld + c] = qc[i] + alpha * delta[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------ k_flux
+enum { MODE_RHS = 0, MODE_EULER = 1, MODE_STAGE = 2, MODE_FINAL = 3 };
+
+template <int MAXF, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
+                                                int stage, const T* __restrict__ QF,
+                                                const T* __restrict__ As, const T* In, T* Out,
+                                                const T* __restrict__ Ag, int wb) {
+    if (!ctrl->active) return;
+    const int c = cell_index();
+    if (c >= m.ncells) return;
+    const int ld = m.ld;
+    const T time = ctrl->stage_t[stage];
+    T fc[NQ], ac[NA];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) fc[i] = 0;
+    load_soa<NA>(As, ld, c, ac);
+    T qcell[NQ];   // clamped cell state (only read by the well-balanced term)
+    if constexpr (HasCellSource<T>::value) {
+        if (wb) {
+            T a0[NA];
+            load_soa<NQ>(In, ld, c, qcell);
+            load_soa<NA>(Ag, ld, c, a0);
+            auto rq = M::update_variables(qcell, a0, P.p, 0.0);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) {
+        const int f = m.cface[(size_t)k * ld + c];
+        if (f < 0) continue;
+        const T area = m.farea[f];
+        if (area <= 1e-15) continue;
+        const int nb = m.cnbr[(size_t)k * ld + c];
+        const T nh[3] = {m.fnhx[f], DIM > 1 ? m.fnhy[f] : (T)0, 0};
+        const T xf[3] = {m.fcx[f], DIM > 1 ? m.fcy[f] : (T)0, 0};
+        T qo[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) qo[i] = QF[((size_t)k * NQ + i) * ld + c];
+        if (nb >= 0) {
+            const int meta = m.cmeta[(size_t)k * ld + c];
+            const int ks = meta & 3;
+            const bool right = (meta >> 2) & 1;
+            T qn[NQ], an[NA];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
+            load_soa<NA>(As, ld, nb, an);
+            const T* qL = right ? qn : qo;
+            const T* qR = right ? qo : qn;
+            const T* aLc = right ? an : ac;
+            const T* aRc = right ? ac : an;
+            auto aL = M::update_aux_variables(qL, aLc, P.p, 0.0, xf);
+            auto aR = M::update_aux_variables(qR, aRc, P.p, 0.0, xf);
+            auto F = NUM::numerical_flux(qL, qR, aL.data, aR.data, P.p, nh);
+            auto D = NUM::numerical_fluctuations(qL, qR, aL.data, aR.data, P.p, nh);
+            if (right) {
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    fc[i] -= D[i] * area;
+                    fc[i] += F[i] * area;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    fc[i] -= D[NQ + i] * area;
+                    fc[i] -= F[i] * area;
+                }
+            }
+        } else {
+            const int bc = m.fbc[f];
+            if (bc >= 0) {
+                auto aL = M::update_aux_variables(qo, ac, P.p, time, xf);
+                auto qR = M::boundary_conditions(bc, qo, aL.data, P.p, nh, xf, time, 0.0);
+                auto aR = M::update_aux_variables(qR.data, aL.data, P.p, time, xf);
+                auto F = NUM::numerical_flux(qo, qR.data, aL.data, aR.data, P.p, nh);
+                auto D = NUM::numerical_fluctuations(qo, qR.data, aL.data, aR.data, P.p, nh);
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    fc[i] -= D[NQ + i] * area;
+                    fc[i] -= F[i] * area;
+                }
+            }
+        }
+        if constexpr (HasCellSource<T>::value) {
+            if (wb) {
+                const bool right = nb >= 0 && ((m.cmeta[(size_t)k * ld + c] >> 2) & 1);
+                const T no[3] = {right ? -nh[0] : nh[0], right ? -nh[1] : nh[1], 0};
+                auto S = M::hydrostatic_cell_source(qo, qcell, P.p, no);
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) fc[i] += S[i] * area;
+            }
+        }
+    }
+    const T vol = m.vol[c];
+    if (vol > 1e-15) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) fc[i] /= vol;
+    }
+    if constexpr (MODE == MODE_RHS) {
+        store_soa<NQ>(Out, ld, c, fc);
+    } else if constexpr (MODE == MODE_EULER || MODE == MODE_STAGE) {
+        // VecAXPY(u, a, F): u += a*F   (Euler: a = dt; SSP stage: a = dt/(s-1))
+        const T a = ctrl->a_stage;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) Out[(size_t)i * ld + c] = In[(size_t)i * ld + c] + a * fc[i];
+    } else {
+        // VecAXPBYPCZ(X, (s-1)/s, dt/s, 1/s, W, F)
+        const T al = ctrl->al, be = ctrl->be, ga = ctrl->ga;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+            Out[(size_t)i * ld + c] = al * In[(size_t)i * ld + c] + be * fc[i] + ga * Out[(size_t)i * ld + c];
+    }
+}
+
+// ------------------------------------------------------------------------------ source
+// SourceStep::SolveLinearSystem (GEPP) with register-resident rows: the pivot swap is a
+// compile-time-indexed select so the n x n matrix never spills.
+__device__ __forceinline__ bool gepp_reg(T* A, T* b, T* x) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        int mr = k;
+        T mv = std::fabs(A[k * NQ + k]);
+#pragma unroll
+        for (int i = k + 1; i < NQ; ++i) {
+            const T v = std::fabs(A[i * NQ + k]);
+            if (v > mv) {
+                mv = v;
+                mr = i;
+            }
+        }
+        if (mv < 1e-14) return false;
+#pragma unroll
+        for (int i = k + 1; i < NQ; ++i) {
+            if (i == mr) {
+#pragma unroll
+                for (int j = k; j < NQ; ++j) {
+                    const T t = A[k * NQ + j];
+                    A[k * NQ + j] = A[i * NQ + j];
+                    A[i * NQ + j] = t;
+                }
+                const T t = b[k];
+                b[k] = b[i];
+                b[i] = t;
+            }
+        }
+#pragma unroll
+        for (int i = k + 1; i < NQ; ++i) {
+            const T fac = A[i * NQ + k] / A[k * NQ + k];
+#pragma unroll
+            for (int j = k; j < NQ; ++j) A[i * NQ + j] -= fac * A[k * NQ + j];
+            b[i] -= fac * b[k];
+        }
+    }
+#pragma unroll
+    for (int i = NQ - 1; i >= 0; --i) {
+        T sum = 0.0;
+#pragma unroll
+        for (int j = i + 1; j < NQ; ++j) sum += A[i * NQ + j] * x[j];
+        x[i] = (b[i] - sum) / A[i * NQ + i];
+    }
+    return true;
+}
+
+// SourceStep::SolveLocalCell (SourceStep.hpp:125-209): implicit Euler Newton, <= 15
+// iterations, |R|_2 < 1e-8, aux Jacobian by forward differences (SourceStep.hpp:40-50)
+__device__ __forceinline__ void newton_cell(const Params& P, T dt, T* q, const T* aux_init) {
+    const T X0[3] = {0, 0, 0};
+    T qo[NQ], a[NA];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) qo[i] = q[i];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) a[i] = aux_init[i];
+    for (int it = 0; it < 15; ++it) {
+        auto ua = M::update_aux_variables(q, a, P.p, 0.0, X0);
+#pragma unroll
+        for (int i = 0; i < NA; ++i) a[i] = ua[i];
+        auto S = M::source(q, a, P.p, (T)0, dt, X0);
+        T r[NQ];
+        T rn = 0.0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            r[i] = q[i] - qo[i] - dt * S[i];
+            rn += r[i] * r[i];
+        }
+        if (std::sqrt(rn) < 1e-8) break;
+        auto dSdQ = M::source_jacobian_wrt_variables(q, a, P.p);
+        auto dSdA = M::source_jacobian_wrt_aux_variables(q, a, P.p);
+        T Jx[NA * NQ];
+        {
+            auto a0 = M::update_aux_variables(q, a, P.p, 0.0, X0);
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                T qp[NQ];
+#pragma unroll
+                for (int l = 0; l < NQ; ++l) qp[l] = q[l];
+                const T h = (T)1e-7 * (std::fabs(q[j]) + (T)1e-7);
+                qp[j] += h;
+                auto a1 = M::update_aux_variables(qp, a, P.p, 0.0, X0);
+#pragma unroll
+                for (int l = 0; l < NA; ++l) Jx[l * NQ + j] = (a1[l] - a0[l]) / h;
+            }
+        }
+        T J[NQ * NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                T term = dSdQ[i * NQ + j];
+#pragma unroll
+                for (int l = 0; l < NA; ++l) term += dSdA[i * NA + l] * Jx[l * NQ + j];
+                J[i * NQ + j] = (i == j ? (T)1.0 : (T)0.0) - dt * term;
+            }
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) r[i] = -r[i];
+        T dl[NQ];
+        if (gepp_reg(J, r, dl)) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) q[i] += dl[i];
+        } else {
+            break;
+        }
+    }
+}
+
+// local CFL ratio r_c / lambda_c (VirtualSolver.hpp:316-345); DBL_MAX when wave-free
+__device__ __forceinline__ T cfl_ratio(const Params& P, const T* q, const T* a, T rc) {
+    T eig = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        T n[3] = {0, 0, 0};
+        n[d] = 1.0;
+        auto r = NUM::local_max_abs_eigenvalue(q, a, P.p, n);
+        if (r[0] > eig) eig = r[0];
+    }
+    if (eig < 1e-14) return DBL_MAX;
+    if (rc < DBL_MAX) return rc / eig;
+    return DBL_MAX;
+}
+
+__device__ __forceinline__ T block_min(T v) {
+    __shared__ T red[BLOCK / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = red[i] < v ? red[i] : v;
+    }
+    return v;
+}
+
+// POST: 0 = CFL only, 1 = constraints + CFL, 2 = constraints + source + constraints + CFL,
+//       3 = source only (no CFL), 4 = constraints only (no CFL)
+template <int POST>
+__global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl* __restrict__ ctrl, T* X,
+                                                T* A, T* __restrict__ partials, T dt_override) {
+    if (!ctrl->active) return;
+    const int c = cell_index();
+    const int ld = m.ld;
+    T ratio = DBL_MAX;
+    if (c < m.ncells) {
+        T q[NQ], a[NA];
+        load_soa<NQ>(X, ld, c, q);
+        load_soa<NA>(A, ld, c, a);
+        if constexpr (POST == 1 || POST == 2 || POST == 4) update_state_cell(P, q, a);
+        if constexpr (POST == 2 || POST == 3) {
+            const T dt = POST == 2 ? ctrl->dt : dt_override;
+            newton_cell(P, dt, q, a);
+        }
+        if constexpr (POST == 2) update_state_cell(P, q, a);
+        if constexpr (POST != 0) {
+            store_soa<NQ>(X, ld, c, q);
+            store_soa<NA>(A, ld, c, a);
+        }
+        if constexpr (POST <= 2) ratio = cfl_ratio(P, q, a, m.rc[c]);
+    }
+    if constexpr (POST <= 2) {
+        ratio = block_min(ratio);
+        if (threadIdx.x == 0) partials[blockIdx.x] = ratio;
+    }
+}
+
+// what: 0 = standalone ComputeTimeStep (-> dt_query), 1 = first dt, 2 = end of a step
+__global__ __launch_bounds__(BLOCK) void k_finalize(Ctrl* ctrl, const T* __restrict__ partials, int np,
+                                                    int what, int order, int s, T cfl, T min_dt,
+                                                    T t_end, T dt_max) {
+    if (what == 2 && !ctrl->active) return;
+    T v = DBL_MAX;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v = partials[i] < v ? partials[i] : v;
+    v = block_min(v);
+    if (threadIdx.x != 0) return;
+    const bool fin = !(!isfinite(v) || v >= 0.5 * DBL_MAX);
+    const T dtc = fin ? cfl * v : dt_max;
+    if (what == 0) {
+        ctrl->dt_query = dtc;
+        ctrl->finite = fin;
+        return;
+    }
+    if (what == 1) {
+        ctrl->dt_cfl = (dtc < min_dt) ? min_dt : dtc;   // std::max(ComputeTimeStep(), min_dt)
+    } else {
+        ctrl->t = ctrl->t + ctrl->dt;
+        ctrl->steps = ctrl->steps + 1;
+        ctrl->dt_cfl = dtc;
+    }
+    ctrl->finite = fin;
+    // prepare the next step (MUSCLSolver.hpp:140-149)
+    const T t0 = ctrl->t;
+    ctrl->active = t0 < t_end;
+    T dt0 = ctrl->dt_cfl;
+    if (t0 + dt0 > t_end) dt0 = t_end - t0;
+    ctrl->dt = dt0;
+    if (order < 2) {
+        ctrl->a_stage = dt0;
+        ctrl->stage_t[0] = t0;
+    } else {
+        ctrl->a_stage = dt0 / (s - 1.);
+        for (int i = 0; i < s - 1; ++i) ctrl->stage_t[i] = t0 + dt0 * (i / (s - 1.));
+        ctrl->stage_t[s - 1] = t0 + dt0;
+        ctrl->al = (s - 1.) / s;
+        ctrl->be = dt0 / s;
+        ctrl->ga = 1. / s;
+    }
+}
+
+__global__ void k_aos_to_soa(const T* __restrict__ aos, T* __restrict__ soa, int n, int nc, int ld) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int i = 0; i < nc; ++i) soa[(size_t)i * ld + c] = aos[(size_t)c * nc + i];
+}
+__global__ void k_soa_to_aos(const T* __restrict__ soa, T* __restrict__ aos, int n, int nc, int ld) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int i = 0; i < nc; ++i) aos[(size_t)c * nc + i] = soa[(size_t)i * ld + c];
+}
+__global__ void k_fill(T* __restrict__ a, size_t n, T v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+}  // namespace
+
+// ================================================================================ host side
+
+struct zdm_solver {
+    zdm_options opt;
+    Params P;
+    int ncells, nfaces, ld, maxf, nblocks;
+    hipStream_t stream;
+    char* arena;
+    size_t arena_bytes;
+    DevMesh dm;
+    T *X, *W, *F, *A, *As, *Atmp, *QF, *partials, *stage_aos;
+    Ctrl *ctrl, *ctrl_rhs;
+    bool timing;
+    std::vector<hipEvent_t> evpool;   // recorded in order; pending until resolved
+    std::vector<int> evkind;          // kernel id that ends at event i (-1 = segment start)
+    size_t evused;
+    double kt_ms[3];
+    int kt_n[3];
+};
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Least-squares weights of LeastSquaresGradient::Setup (Gradient.hpp:222-298), order 1
+bool chol_solve_host(int n, std::vector<T>& A, std::vector<T>& b) {
+    std::vector<T> L(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j <= i; j++) {
+            T sum = 0;
+            for (int k = 0; k < j; k++) sum += L[i * n + k] * L[j * n + k];
+            if (i == j) {
+                T val = A[i * n + i] - sum;
+                if (val <= 0) return false;
+                L[i * n + j] = std::sqrt(val);
+            } else {
+                L[i * n + j] = (A[i * n + j] - sum) / L[j * n + j];
+            }
+        }
+    for (int i = 0; i < n; i++) {
+        T sum = 0;
+        for (int k = 0; k < i; k++) sum += L[i * n + k] * b[k];
+        b[i] = (b[i] - sum) / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        T sum = 0;
+        for (int k = i + 1; k < n; k++) sum += L[k * n + i] * b[k];
+        b[i] = (b[i] - sum) / L[i * n + i];
+    }
+    return true;
+}
+
+template <int MAXF>
+void launch_prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag) {
+    dim3 g(s->nblocks), b(BLOCK);
+    const int lim = s->opt.use_limiters;
+    if (s->opt.order < 2)
+        k_prep<MAXF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, s->As, s->QF, lim);
+    else if (s->opt.gradient == 1)
+        k_prep<MAXF, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, s->As, s->QF, lim);
+    else
+        k_prep<MAXF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, s->As, s->QF, lim);
+}
+
+template <int MAXF>
+void launch_flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag) {
+    dim3 g(s->nblocks), b(BLOCK);
+    const int wb = s->opt.well_balanced;
+    switch (mode) {
+        case MODE_RHS:
+            k_flux<MAXF, MODE_RHS><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            break;
+        case MODE_EULER:
+            k_flux<MAXF, MODE_EULER><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            break;
+        case MODE_STAGE:
+            k_flux<MAXF, MODE_STAGE><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            break;
+        default:
+            k_flux<MAXF, MODE_FINAL><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            break;
+    }
+}
+
+void prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag) {
+    if (s->maxf <= 2)
+        launch_prep<2>(s, ctrl, Xin, Ag);
+    else if (s->maxf == 3)
+        launch_prep<3>(s, ctrl, Xin, Ag);
+    else
+        launch_prep<4>(s, ctrl, Xin, Ag);
+}
+
+void flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag) {
+    if (s->maxf <= 2)
+        launch_flux<2>(s, ctrl, mode, stage, In, Out, Ag);
+    else if (s->maxf == 3)
+        launch_flux<3>(s, ctrl, mode, stage, In, Out, Ag);
+    else
+        launch_flux<4>(s, ctrl, mode, stage, In, Out, Ag);
+}
+
+void post(zdm_solver* s, int mode, T dt_override = 0) {
+    dim3 g(s->nblocks), b(BLOCK);
+    switch (mode) {
+        case 0: k_post<0><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
+        case 1: k_post<1><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
+        case 2: k_post<2><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
+        case 3: k_post<3><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl_rhs, s->X, s->A, s->partials, dt_override); break;
+        default: k_post<4><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl_rhs, s->X, s->A, s->partials, dt_override); break;
+    }
+}
+
+void finalize(zdm_solver* s, int what) {
+    k_finalize<<<1, BLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, what, s->opt.order,
+                                          s->opt.ssp_stages, s->opt.cfl, s->opt.min_dt, s->opt.t_end,
+                                          (T)M::dt_max);
+}
+
+int set_ctrl_rhs_time(zdm_solver* s, T t) {
+    Ctrl c;
+    std::memset(&c, 0, sizeof(c));
+    c.active = 1;
+    c.t = t;
+    c.stage_t[0] = t;
+    HIPCHK(hipMemcpyAsync(s->ctrl_rhs, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* zdm_last_error(void) { return g_err.c_str(); }
+const char* zdm_model_name(void) { return ZDM_MODEL_NAME; }
+
+int zdm_model_info(int* n_q, int* n_aux, int* n_params, int* dim) {
+    if (n_q) *n_q = NQ;
+    if (n_aux) *n_aux = NA;
+    if (n_params) *n_params = NPAR;
+    if (dim) *dim = DIM;
+    return 0;
+}
+
+int zdm_model_default_params(double* params) {
+    auto v = M::default_parameters();
+    for (size_t i = 0; i < v.size(); ++i) params[i] = v[i];
+    return 0;
+}
+
+int zdm_model_boundary_tags(char* names64, int cap) {
+    auto v = M::get_boundary_tags();
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) {
+        std::memset(names64 + 64 * i, 0, 64);
+        std::strncpy(names64 + 64 * i, v[i].c_str(), 63);
+    }
+    return (int)v.size();
+}
+
+int zdm_model_parameter_names(char* names64, int cap) {
+    auto v = M::parameter_names();
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) {
+        std::memset(names64 + 64 * i, 0, 64);
+        std::strncpy(names64 + 64 * i, v[i].c_str(), 63);
+    }
+    return (int)v.size();
+}
+
+int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* params,
+                      const zdm_options* opts, zdm_solver** out) {
+    if (!mesh || !face_bc || !opts || !out) return fail("zdm_solver_create: null argument");
+    if (mesh->dim != DIM) return fail("mesh dimension does not match Model::dimension");
+    if (mesh->max_faces > 4 || mesh->max_faces < 1) return fail("unsupported faces per cell");
+    if (opts->order >= 2 && opts->ssp_stages < 2) return fail("ssp_stages must be >= 2");
+    if (opts->ssp_stages > 16) return fail("ssp_stages must be <= 16");
+    if (opts->block_size != 0 && opts->block_size != BLOCK) return fail("block_size must be 256");
+    if (opts->well_balanced && !HasCellSource<T>::value)
+        return fail("well_balanced needs Model::hydrostatic_cell_source");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
+
+    zdm_solver* s = new zdm_solver();
+    std::memset(&s->P, 0, sizeof(Params));
+    s->opt = *opts;
+    auto dp = M::default_parameters();
+    for (int i = 0; i < NPAR; ++i) s->P.p[i] = params ? params[i] : dp[i];
+    const int nc = mesh->ncells, nf = mesh->nfaces, MF = mesh->max_faces;
+    s->ncells = nc;
+    s->nfaces = nf;
+    s->maxf = MF;
+    s->ld = (int)align_up((size_t)nc, 256);
+    s->nblocks = (nc + BLOCK - 1) / BLOCK;
+    const int ld = s->ld;
+    const int MFa = MF <= 2 ? 2 : MF;
+
+    // ---- host-side geometry tables ----
+    std::vector<T> vol(ld, 0), cx(ld, 0), cy(ld, 0), rc(ld, DBL_MAX);
+    std::vector<int> cface((size_t)MFa * ld, -1), cnbr((size_t)MFa * ld, -1), cmeta((size_t)MFa * ld, 0);
+    std::vector<T> fNx(nf), fNy(nf), fcx(nf), fcy(nf), farea(nf), fnhx(nf), fnhy(nf);
+    std::vector<int> fbc(nf);
+    std::vector<T> lsqw;
+    for (int f = 0; f < nf; ++f) {
+        const double* N = mesh->face_normal + 3 * (size_t)f;
+        fNx[f] = N[0];
+        fNy[f] = DIM > 1 ? N[1] : 0;
+        fcx[f] = mesh->face_centroid[3 * (size_t)f];
+        fcy[f] = DIM > 1 ? mesh->face_centroid[3 * (size_t)f + 1] : 0;
+        T area = 0;
+        for (int d = 0; d < DIM; ++d) area += N[d] * N[d];
+        area = std::sqrt(area);
+        farea[f] = area;
+        fnhx[f] = N[0] / area;
+        fnhy[f] = DIM > 1 ? N[1] / area : 0;
+        fbc[f] = mesh->face_cells[2 * f + 1] >= 0 ? -1 : face_bc[f];
+    }
+    for (int c = 0; c < nc; ++c) {
+        vol[c] = mesh->cell_volume[c];
+        cx[c] = mesh->cell_centroid[3 * (size_t)c];
+        cy[c] = DIM > 1 ? mesh->cell_centroid[3 * (size_t)c + 1] : 0;
+        T r = DBL_MAX;
+        for (int k = 0; k < mesh->cell_nfaces[c]; ++k) {
+            const int f = mesh->cell_faces[(size_t)c * MF + k];
+            const int nb = mesh->cell_nbrs[(size_t)c * MF + k];
+            cface[(size_t)k * ld + c] = f;
+            cnbr[(size_t)k * ld + c] = nb;
+            int meta = (mesh->face_cells[2 * f] == c) ? 0 : 4;
+            if (nb >= 0) {
+                int ks = -1;
+                for (int j = 0; j < mesh->cell_nfaces[nb]; ++j)
+                    if (mesh->cell_faces[(size_t)nb * MF + j] == f) ks = j;
+                if (ks < 0) {
+                    delete s;
+                    return fail("inconsistent mesh: face missing from its neighbour");
+                }
+                meta |= ks;
+            }
+            cmeta[(size_t)k * ld + c] = meta;
+            const double* xf = mesh->face_centroid + 3 * (size_t)f;
+            T d2 = 0.0;
+            for (int d = 0; d < DIM; ++d) {
+                T dd = mesh->cell_centroid[3 * (size_t)c + d] - xf[d];
+                d2 += dd * dd;
+            }
+            T dist = std::sqrt(d2);
+            if (dist < r) r = dist;
+        }
+        rc[c] = r;
+    }
+    if (opts->gradient == 1) {
+        lsqw.assign((size_t)MFa * DIM * ld, 0.0);
+        for (int c = 0; c < nc; ++c) {
+            const double* xc = mesh->cell_centroid + 3 * (size_t)c;
+            std::vector<int> slots;
+            std::vector<std::vector<T>> bv;
+            for (int k = 0; k < mesh->cell_nfaces[c]; ++k) {
+                const int nb = mesh->cell_nbrs[(size_t)c * MF + k];
+                if (nb < 0) continue;
+                const double* xn = mesh->cell_centroid + 3 * (size_t)nb;
+                std::vector<T> b(DIM);
+                for (int d = 0; d < DIM; ++d) b[d] = xn[d] - xc[d];
+                slots.push_back(k);
+                bv.push_back(b);
+            }
+            std::vector<T> ata(DIM * DIM, 0.0);
+            for (const auto& b : bv)
+                for (int i = 0; i < DIM; ++i)
+                    for (int j = 0; j < DIM; ++j) ata[i * DIM + j] += b[i] * b[j];
+            for (size_t j = 0; j < slots.size(); ++j) {
+                std::vector<T> w = bv[j], sys = ata;
+                if (!chol_solve_host(DIM, sys, w)) std::fill(w.begin(), w.end(), 0.0);
+                for (int d = 0; d < DIM; ++d) lsqw[((size_t)slots[j] * DIM + d) * ld + c] = w[d];
+            }
+        }
+    }
+
+    // ---- one device arena ----
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const size_t sQ = sizeof(T) * (size_t)NQ * ld, sA = sizeof(T) * (size_t)NA * ld;
+    size_t oX = take(sQ), oW = take(sQ), oF = take(sQ), oA = take(sA), oAs = take(sA), oAt = take(sA);
+    size_t oQF = take(sizeof(T) * (size_t)MFa * NQ * ld);
+    size_t oPart = take(sizeof(T) * (size_t)s->nblocks);
+    size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
+    size_t oCtrl = take(sizeof(Ctrl)), oCtrlR = take(sizeof(Ctrl));
+    size_t oVol = take(sizeof(T) * ld), oCx = take(sizeof(T) * ld), oCy = take(sizeof(T) * ld), oRc = take(sizeof(T) * ld);
+    size_t oCface = take(sizeof(int) * (size_t)MFa * ld), oCnbr = take(sizeof(int) * (size_t)MFa * ld),
+           oCmeta = take(sizeof(int) * (size_t)MFa * ld);
+    size_t oF7[7];
+    for (int i = 0; i < 7; ++i) oF7[i] = take(sizeof(T) * (size_t)std::max(nf, 1));
+    size_t oFbc = take(sizeof(int) * (size_t)std::max(nf, 1));
+    size_t oLsq = take(sizeof(T) * (lsqw.empty() ? 1 : lsqw.size()));
+    s->arena_bytes = off;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail("hipStreamCreate failed");
+    }
+    if (hipMalloc(&s->arena, s->arena_bytes) != hipSuccess) {
+        hipStreamDestroy(s->stream);
+        delete s;
+        return fail("hipMalloc of the solver arena failed");
+    }
+    char* a = s->arena;
+    s->X = (T*)(a + oX);
+    s->W = (T*)(a + oW);
+    s->F = (T*)(a + oF);
+    s->A = (T*)(a + oA);
+    s->As = (T*)(a + oAs);
+    s->Atmp = (T*)(a + oAt);
+    s->QF = (T*)(a + oQF);
+    s->partials = (T*)(a + oPart);
+    s->stage_aos = (T*)(a + oStage);
+    s->ctrl = (Ctrl*)(a + oCtrl);
+    s->ctrl_rhs = (Ctrl*)(a + oCtrlR);
+    auto up = [&](size_t o, const void* src, size_t bytes) {
+        return hipMemcpy(a + o, src, bytes, hipMemcpyHostToDevice);
+    };
+    hipError_t e = hipSuccess;
+    e = e != hipSuccess ? e : hipMemset(a, 0, s->arena_bytes);
+    e = e != hipSuccess ? e : up(oVol, vol.data(), sizeof(T) * ld);
+    e = e != hipSuccess ? e : up(oCx, cx.data(), sizeof(T) * ld);
+    e = e != hipSuccess ? e : up(oCy, cy.data(), sizeof(T) * ld);
+    e = e != hipSuccess ? e : up(oRc, rc.data(), sizeof(T) * ld);
+    e = e != hipSuccess ? e : up(oCface, cface.data(), sizeof(int) * cface.size());
+    e = e != hipSuccess ? e : up(oCnbr, cnbr.data(), sizeof(int) * cnbr.size());
+    e = e != hipSuccess ? e : up(oCmeta, cmeta.data(), sizeof(int) * cmeta.size());
+    const std::vector<T>* fa[7] = {&fNx, &fNy, &fcx, &fcy, &farea, &fnhx, &fnhy};
+    for (int i = 0; i < 7 && nf > 0; ++i) e = e != hipSuccess ? e : up(oF7[i], fa[i]->data(), sizeof(T) * nf);
+    if (nf > 0) e = e != hipSuccess ? e : up(oFbc, fbc.data(), sizeof(int) * nf);
+    if (!lsqw.empty()) e = e != hipSuccess ? e : up(oLsq, lsqw.data(), sizeof(T) * lsqw.size());
+    if (e != hipSuccess) {
+        hipFree(s->arena);
+        hipStreamDestroy(s->stream);
+        delete s;
+        return fail(std::string("device upload failed: ") + hipGetErrorString(e));
+    }
+    DevMesh& d = s->dm;
+    d.ncells = nc;
+    d.nfaces = nf;
+    d.ld = ld;
+    d.vol = (T*)(a + oVol);
+    d.cx = (T*)(a + oCx);
+    d.cy = (T*)(a + oCy);
+    d.rc = (T*)(a + oRc);
+    d.cface = (int*)(a + oCface);
+    d.cnbr = (int*)(a + oCnbr);
+    d.cmeta = (int*)(a + oCmeta);
+    d.fNx = (T*)(a + oF7[0]);
+    d.fNy = (T*)(a + oF7[1]);
+    d.fcx = (T*)(a + oF7[2]);
+    d.fcy = (T*)(a + oF7[3]);
+    d.farea = (T*)(a + oF7[4]);
+    d.fnhx = (T*)(a + oF7[5]);
+    d.fnhy = (T*)(a + oF7[6]);
+    d.fbc = (int*)(a + oFbc);
+    d.lsqw = (T*)(a + oLsq);
+    s->timing = false;
+    s->evused = 0;
+    for (int i = 0; i < 3; ++i) {
+        s->kt_ms[i] = 0;
+        s->kt_n[i] = 0;
+    }
+    Ctrl c0;
+    std::memset(&c0, 0, sizeof(c0));
+    hipMemcpy(s->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice);
+    *out = s;
+    return 0;
+}
+
+int zdm_solver_destroy(zdm_solver* s) {
+    if (!s) return 0;
+    hipStreamSynchronize(s->stream);
+    for (hipEvent_t e : s->evpool) hipEventDestroy(e);
+    hipFree(s->arena);
+    hipStreamDestroy(s->stream);
+    delete s;
+    return 0;
+}
+
+static int upload_soa(zdm_solver* s, const double* host_aos, T* dst, int ncomp, bool from_device) {
+    const size_t bytes = sizeof(T) * (size_t)s->ncells * ncomp;
+    if (from_device)
+        HIPCHK(hipMemcpyAsync(s->stage_aos, host_aos, bytes, hipMemcpyDeviceToDevice, s->stream));
+    else
+        HIPCHK(hipMemcpyAsync(s->stage_aos, host_aos, bytes, hipMemcpyHostToDevice, s->stream));
+    k_aos_to_soa<<<(s->ncells + 255) / 256, 256, 0, s->stream>>>(s->stage_aos, dst, s->ncells, ncomp, s->ld);
+    HIPCHK(hipGetLastError());
+    if (!from_device) HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+static int download_soa(zdm_solver* s, const T* src, double* host_aos, int ncomp, bool to_device) {
+    const size_t bytes = sizeof(T) * (size_t)s->ncells * ncomp;
+    k_soa_to_aos<<<(s->ncells + 255) / 256, 256, 0, s->stream>>>(src, s->stage_aos, s->ncells, ncomp, s->ld);
+    HIPCHK(hipGetLastError());
+    if (to_device)
+        HIPCHK(hipMemcpyAsync(host_aos, s->stage_aos, bytes, hipMemcpyDeviceToDevice, s->stream));
+    else {
+        HIPCHK(hipMemcpyAsync(host_aos, s->stage_aos, bytes, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    return 0;
+}
+
+static int fill_zero(zdm_solver* s, T* dst, size_t n) {
+    k_fill<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(dst, n, 0.0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int zdm_state_upload(zdm_solver* s, const double* X, const double* A) {
+    if (!s || !X) return fail("zdm_state_upload: null argument");
+    if (upload_soa(s, X, s->X, NQ, false)) return -1;
+    if (A) return upload_soa(s, A, s->A, NA, false);
+    if (fill_zero(s, s->A, (size_t)NA * s->ld)) return -1;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int zdm_state_download(zdm_solver* s, double* X, double* A) {
+    if (!s) return fail("zdm_state_download: null argument");
+    if (X && download_soa(s, s->X, X, NQ, false)) return -1;
+    if (A && download_soa(s, s->A, A, NA, false)) return -1;
+    return 0;
+}
+
+int zdm_state_upload_device(zdm_solver* s, const double* dX, const double* dA) {
+    if (!s || !dX) return fail("zdm_state_upload_device: null argument");
+    if (upload_soa(s, dX, s->X, NQ, true)) return -1;
+    if (dA) return upload_soa(s, dA, s->A, NA, true);
+    return fill_zero(s, s->A, (size_t)NA * s->ld);
+}
+
+int zdm_state_download_device(zdm_solver* s, double* dX, double* dA) {
+    if (!s) return fail("zdm_state_download_device: null argument");
+    if (dX && download_soa(s, s->X, dX, NQ, true)) return -1;
+    if (dA && download_soa(s, s->A, dA, NA, true)) return -1;
+    return 0;
+}
+
+int zdm_initialize(zdm_solver* s) {
+    if (!s) return fail("null solver");
+    Ctrl c0;
+    std::memset(&c0, 0, sizeof(c0));
+    c0.active = 1;
+    HIPCHK(hipMemcpyAsync(s->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    post(s, 1);
+    finalize(s, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+static void record(zdm_solver* s, int kind) {
+    if (!s->timing) return;
+    if (s->evused == s->evpool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        s->evpool.push_back(e);
+        s->evkind.push_back(-1);
+    }
+    s->evkind[s->evused] = kind;
+    hipEventRecord(s->evpool[s->evused], s->stream);
+    s->evused++;
+}
+
+// Fold the recorded events into per-kernel totals (synchronises the stream).
+static int resolve_events(zdm_solver* s) {
+    if (s->evused == 0) return 0;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t i = 1; i < s->evused; ++i) {
+        const int k = s->evkind[i];
+        if (k < 0) continue;
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, s->evpool[i - 1], s->evpool[i]));
+        s->kt_ms[k] += ms;
+        s->kt_n[k] += 1;
+    }
+    s->evused = 0;
+    return 0;
+}
+
+int zdm_step(zdm_solver* s, int n) {
+    if (!s) return fail("null solver");
+    const int order = s->opt.order, st = s->opt.ssp_stages;
+    for (int step = 0; step < n; ++step) {
+        if (s->timing && s->evpool.size() > 4096 && s->evused + 64 > s->evpool.size())
+            if (resolve_events(s)) return -1;
+        record(s, -1);
+        if (order < 2) {
+            prep(s, s->ctrl, s->X, s->A);
+            record(s, 0);
+            flux(s, s->ctrl, MODE_EULER, 0, s->X, s->X, s->A);
+            record(s, 1);
+        } else {
+            for (int i = 0; i < st - 1; ++i) {
+                const T* in = i == 0 ? s->X : s->W;
+                prep(s, s->ctrl, in, s->A);
+                record(s, 0);
+                flux(s, s->ctrl, MODE_STAGE, i, in, s->W, s->A);
+                record(s, 1);
+            }
+            prep(s, s->ctrl, s->W, s->A);
+            record(s, 0);
+            flux(s, s->ctrl, MODE_FINAL, st - 1, s->W, s->X, s->A);
+            record(s, 1);
+        }
+        post(s, s->opt.source ? 2 : 1);
+        record(s, 2);
+        finalize(s, 2);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+int zdm_enable_timing(zdm_solver* s, int on) {
+    if (!s) return fail("null solver");
+    if (resolve_events(s)) return -1;
+    s->timing = on != 0;
+    for (int i = 0; i < 3; ++i) {
+        s->kt_ms[i] = 0;
+        s->kt_n[i] = 0;
+    }
+    return 0;
+}
+
+int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches) {
+    if (!s || kernel < 0 || kernel > 2) return fail("bad kernel id");
+    if (resolve_events(s)) return -1;
+    *launches = s->kt_n[kernel];
+    *avg_ms = s->kt_n[kernel] ? s->kt_ms[kernel] / s->kt_n[kernel] : 0.0;
+    return 0;
+}
+
+int zdm_get_time(zdm_solver* s, double* time, double* dt, int* steps) {
+    if (!s) return fail("null solver");
+    Ctrl c;
+    HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (time) *time = c.t;
+    if (dt) *dt = c.dt_cfl;
+    if (steps) *steps = c.steps;
+    return 0;
+}
+
+int zdm_run(zdm_solver* s, int max_steps, int check_every, int* steps_done) {
+    if (!s) return fail("null solver");
+    if (check_every < 1) check_every = 1;
+    int done = 0, st0 = 0;
+    if (zdm_get_time(s, nullptr, nullptr, &st0)) return -1;
+    while (done < max_steps) {
+        int n = std::min(check_every, max_steps - done);
+        if (zdm_step(s, n)) return -1;
+        done += n;
+        Ctrl c;
+        HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (!c.active) break;
+    }
+    int st1 = 0;
+    if (zdm_get_time(s, nullptr, nullptr, &st1)) return -1;
+    if (steps_done) *steps_done = st1 - st0;
+    return 0;
+}
+
+int zdm_synchronize(zdm_solver* s) {
+    if (!s) return fail("null solver");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int zdm_transport_rhs(zdm_solver* s, double t, const double* X, const double* A, double* F) {
+    if (!s || !X || !F) return fail("zdm_transport_rhs: null argument");
+    if (upload_soa(s, X, s->W, NQ, false)) return -1;
+    if (A) {
+        if (upload_soa(s, A, s->Atmp, NA, false)) return -1;
+    } else if (fill_zero(s, s->Atmp, (size_t)NA * s->ld)) {
+        return -1;
+    }
+    if (set_ctrl_rhs_time(s, t)) return -1;
+    prep(s, s->ctrl_rhs, s->W, s->Atmp);
+    flux(s, s->ctrl_rhs, MODE_RHS, 0, s->W, s->F, s->Atmp);
+    HIPCHK(hipGetLastError());
+    return download_soa(s, s->F, F, NQ, false);
+}
+
+int zdm_source_step(zdm_solver* s, double dt) {
+    if (!s) return fail("null solver");
+    if (set_ctrl_rhs_time(s, 0.0)) return -1;
+    post(s, 3, dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int zdm_enforce_constraints(zdm_solver* s) {
+    if (!s) return fail("null solver");
+    if (set_ctrl_rhs_time(s, 0.0)) return -1;
+    post(s, 4);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int zdm_cfl_dt(zdm_solver* s, double* dt) {
+    if (!s || !dt) return fail("null argument");
+    // the standalone query must not disturb an inactive run: force the CFL pass on
+    Ctrl c;
+    HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int was = c.active;
+    c.active = 1;
+    HIPCHK(hipMemcpyAsync(s->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    post(s, 0);
+    finalize(s, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    *dt = c.dt_query;
+    c.active = was;
+    HIPCHK(hipMemcpyAsync(s->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+}  // extern "C"
