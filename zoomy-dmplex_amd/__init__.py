@@ -46,7 +46,7 @@ class _ZdmMesh(ctypes.Structure):
         ("cell_verts", _c_int_p), ("face_cells", _c_int_p), ("face_normal", _c_dbl_p),
         ("face_centroid", _c_dbl_p), ("face_tag", _c_int_p), ("verts", _c_dbl_p),
         ("nphysical", ctypes.c_int), ("physical_tag", _c_int_p), ("physical_dim", _c_int_p),
-        ("physical_names", ctypes.c_char_p),
+        ("physical_names", ctypes.c_void_p),
     ]
 
 
