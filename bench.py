@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--grid", type=int, default=1000)
+    ap.add_argument("--order", default="rcm", choices=["hilbert", "rcm", "none"],
+                    help="cell renumbering of the mesh service")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -94,7 +96,7 @@ def main():
     import zoomy_dmplex_amd.cases as cases
 
     n = args.grid
-    mesh = zdm.Mesh.tri_grid(n, n)
+    mesh = zdm.Mesh.tri_grid(n, n, renumber=args.order)
     info = zdm.ModelInfo("sme2")
     X = cases.c2_state(mesh, info.n_q, seed=2)
     s = zdm.Solver("sme2", mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e9, source=True,
@@ -141,6 +143,7 @@ def main():
             "workload": "SME level 2 + implicit slip source, %dx%d right-triangle grid "
                         "(%d cells), MUSCL + Barth-Jespersen, SSP-RK2, CFL 0.45" % (n, n, ncells),
             "cells": ncells,
+            "cell_order": args.order,
             "n_q": info.n_q,
             "faces_per_cell": round(F, 4),
             "step_bytes_per_cell": round(step_bytes_per_cell(info.n_q, F), 1),
@@ -160,6 +163,8 @@ def main():
             "launches": flux_n,
         },
     }
+    copy_gbs, _ = zdm.stream_copy_gbs("sme2", 1 << 30, 20)
+    out["roofline"]["stream_copy_gbs"] = round(copy_gbs, 1)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(zdm, cases)
     s.close()

@@ -14,6 +14,8 @@
 #ifndef ZDM_H
 #define ZDM_H
 
+#include <stddef.h>
+
 #include "zdm_mesh.h"
 
 #ifdef __cplusplus
@@ -96,6 +98,10 @@ int zdm_state_download_device(zdm_solver* s, double* dX, double* dA);
  * 0 = prep (gradient/bounds/reconstruction), 1 = flux + stage update, 2 = post-step. */
 int zdm_enable_timing(zdm_solver* s, int on);
 int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches);
+
+/* Measured HBM bandwidth of a plain device copy of `bytes` (read + write counted), averaged
+ * over `iters` launches: the achievable-peak reference and the traffic-counter calibration. */
+int zdm_stream_copy(size_t bytes, int iters, double* gbs, double* avg_ms);
 
 const char* zdm_last_error(void);
 

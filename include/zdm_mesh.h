@@ -10,8 +10,8 @@
  *   - the "Face Sets" label (boundary tags) from Gmsh $PhysicalNames
  *     (MeshConfigLoader.hpp:15-41, VirtualSolver.hpp:394-436).
  *
- * Cells are renumbered by reverse Cuthill-McKee (optional) so that face neighbours are
- * close in memory; faces are numbered by ascending (support[0], support[1]).
+ * Cells are optionally renumbered (reverse Cuthill-McKee or a Hilbert curve) so that face
+ * neighbours are close in memory; faces are numbered by ascending (support[0], support[1]).
  * Every function returns 0 on success and a negative code on failure; the message is
  * available from zdm_mesh_last_error().
  */

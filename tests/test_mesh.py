@@ -9,9 +9,10 @@ from conftest import ROOT, zdm
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
+@pytest.mark.parametrize("renumber", ["rcm", "hilbert", "none"])
 @pytest.mark.parametrize("n,jitter", [(4, 0.0), (7, 0.15), (16, 0.0)])
-def test_tri_grid_counts_and_geometry(n, jitter):
-    m = zdm.Mesh.tri_grid(n, n, jitter=jitter, seed=9)
+def test_tri_grid_counts_and_geometry(n, jitter, renumber):
+    m = zdm.Mesh.tri_grid(n, n, jitter=jitter, seed=9, renumber=renumber)
     # an n x n split grid has 2n^2 cells, 3n^2 + 2n faces, 4n boundary faces
     assert m.ncells == 2 * n * n
     assert m.nfaces == 3 * n * n + 2 * n
@@ -68,6 +69,15 @@ def test_rcm_reduces_bandwidth():
     inter = m.face_cells[:, 1] >= 0
     bw = np.max(m.face_cells[inter, 1] - m.face_cells[inter, 0])
     assert bw <= 3 * 40 + 4
+
+
+def test_hilbert_order_locality():
+    """Hilbert order keeps face neighbours close: most interior faces join cells less than a
+    wavefront (64 cells) apart."""
+    m = zdm.Mesh.tri_grid(64, 64, renumber="hilbert")
+    inter = m.face_cells[:, 1] >= 0
+    gap = m.face_cells[inter, 1] - m.face_cells[inter, 0]
+    assert np.mean(gap < 64) > 0.9
 
 
 def test_line_mesh():

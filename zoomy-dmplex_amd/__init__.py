@@ -28,7 +28,8 @@ class ZdmError(RuntimeError):
 
 
 def _load(name):
-    path = os.path.join(HERE, name)
+    # ZDM_LIB_DIR selects another build directory of the same libraries (e.g. a tuned build)
+    path = os.path.join(os.environ.get("ZDM_LIB_DIR", HERE), name)
     if not os.path.exists(path):
         raise ZdmError("%s is not built (run zoomy-dmplex_amd/build.py or __graft_entry__.build())"
                        % path)
@@ -72,6 +73,12 @@ def mesh_lib():
         lib.zdm_splitmix64_next.restype = ctypes.c_uint64
         _mesh_lib = lib
     return _mesh_lib
+
+
+def _renum(r):
+    if isinstance(r, str):
+        return {"none": 0, "rcm": 1, "hilbert": 2}[r]
+    return int(r)
 
 
 def _arr(ptr, n, dtype):
@@ -130,18 +137,19 @@ class Mesh:
 
     @classmethod
     def tri_grid(cls, nx, ny=None, jitter=0.0, seed=0, renumber=True):
-        """Unit square, nx x ny squares split SW->NE into right triangles; walls (tag 1)."""
+        """Unit square, nx x ny squares split SW->NE into right triangles; walls (tag 1).
+        renumber: False/0 = none, True/1/"rcm" = reverse Cuthill-McKee, 2/"hilbert"."""
         return cls._make(mesh_lib().zdm_mesh_tri_grid, nx, ny or nx, float(jitter),
-                         ctypes.c_uint64(seed), int(renumber))
+                         ctypes.c_uint64(seed), _renum(renumber))
 
     @classmethod
     def line(cls, n, x0=0.0, x1=1.0, renumber=False):
-        return cls._make(mesh_lib().zdm_mesh_line, n, float(x0), float(x1), int(renumber))
+        return cls._make(mesh_lib().zdm_mesh_line, n, float(x0), float(x1), _renum(renumber))
 
     @classmethod
     def read_msh(cls, path, renumber=True):
         """Gmsh MSH 2.2 (the reference's mesh format, VirtualSolver.hpp:388)."""
-        return cls._make(mesh_lib().zdm_mesh_read_msh, path.encode(), int(renumber))
+        return cls._make(mesh_lib().zdm_mesh_read_msh, path.encode(), _renum(renumber))
 
     @classmethod
     def from_cells(cls, dim, xyz, cells, bfaces=None, btags=None, mark_all_tag=1, renumber=True):
@@ -153,7 +161,7 @@ class Mesh:
         return cls._make(mesh_lib().zdm_mesh_from_cells, dim, xyz.shape[0],
                          xyz.ctypes.data_as(_c_dbl_p), cells.shape[0], cells.shape[1],
                          cells.ctypes.data_as(_c_int_p), nbf, bf.ctypes.data_as(_c_int_p),
-                         bt.ctypes.data_as(_c_int_p), mark_all_tag, int(renumber))
+                         bt.ctypes.data_as(_c_int_p), mark_all_tag, _renum(renumber))
 
 
 def boundary_face_bc(mesh, model_tags, model_dim=None):
@@ -244,7 +252,7 @@ SOLVER_SYMBOLS = [
     "zdm_initialize", "zdm_step", "zdm_run", "zdm_get_time", "zdm_synchronize",
     "zdm_transport_rhs", "zdm_source_step", "zdm_enforce_constraints", "zdm_cfl_dt",
     "zdm_state_upload_device", "zdm_state_download_device", "zdm_enable_timing",
-    "zdm_kernel_time", "zdm_last_error",
+    "zdm_kernel_time", "zdm_stream_copy", "zdm_last_error",
 ]
 
 
@@ -276,6 +284,7 @@ def model_lib(model):
     lib.zdm_cfl_dt.argtypes = [V, _c_dbl_p]
     lib.zdm_enable_timing.argtypes = [V, ctypes.c_int]
     lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
+    lib.zdm_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _c_dbl_p, _c_dbl_p]
     lib.zdm_last_error.restype = ctypes.c_char_p
     _model_libs[model] = lib
     return lib
@@ -435,3 +444,11 @@ class Solver:
         self._check(self.lib.zdm_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+
+def stream_copy_gbs(model="sme2", nbytes=1 << 30, iters=20):
+    """Achievable HBM bandwidth of a plain device copy (GB/s, read + write)."""
+    lib = model_lib(model)
+    g, ms = ctypes.c_double(), ctypes.c_double()
+    if lib.zdm_stream_copy(nbytes, iters, ctypes.byref(g), ctypes.byref(ms)) != 0:
+        raise ZdmError(lib.zdm_last_error().decode())
+    return g.value, ms.value

@@ -80,6 +80,61 @@ std::vector<int> rcm_order(int n, const std::vector<int>& off, const std::vector
     return order;
 }
 
+// Hilbert-curve index of (x, y) on a 2^order x 2^order grid.
+uint64_t hilbert_index(uint32_t order, uint32_t x, uint32_t y) {
+    const uint32_t n = 1u << order;
+    uint64_t d = 0;
+    for (uint32_t s = n >> 1; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+// Cells ordered along a Hilbert curve through their vertex-mean centroids (2-D) or by x
+// (1-D): face neighbours land in the same or adjacent 64-cell wavefronts, which keeps the
+// neighbour gathers of the device kernels inside L2.
+std::vector<int> hilbert_order(int dim, int ncells, int nvpc, const int* cverts,
+                               const std::vector<int>& nvc, const double* xyz) {
+    std::vector<double> cx(ncells), cy(ncells);
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+    for (int c = 0; c < ncells; ++c) {
+        double x = 0, y = 0;
+        for (int i = 0; i < nvc[c]; ++i) {
+            x += xyz[3 * (size_t)cverts[(size_t)c * nvpc + i]];
+            y += xyz[3 * (size_t)cverts[(size_t)c * nvpc + i] + 1];
+        }
+        cx[c] = x / nvc[c];
+        cy[c] = y / nvc[c];
+        lo[0] = std::min(lo[0], cx[c]);
+        hi[0] = std::max(hi[0], cx[c]);
+        lo[1] = std::min(lo[1], cy[c]);
+        hi[1] = std::max(hi[1], cy[c]);
+    }
+    const uint32_t order = 16;
+    const double span = std::max(hi[0] - lo[0], hi[1] - lo[1]);
+    const double scale = span > 0 ? ((1u << order) - 1) / span : 0.0;
+    std::vector<std::pair<uint64_t, int>> key(ncells);
+    for (int c = 0; c < ncells; ++c) {
+        uint32_t ix = (uint32_t)((cx[c] - lo[0]) * scale);
+        uint32_t iy = dim == 2 ? (uint32_t)((cy[c] - lo[1]) * scale) : 0u;
+        key[c] = {dim == 2 ? hilbert_index(order, ix, iy) : (uint64_t)ix, c};
+    }
+    std::stable_sort(key.begin(), key.end());
+    std::vector<int> perm(ncells);
+    for (int c = 0; c < ncells; ++c) perm[c] = key[c].second;
+    return perm;
+}
+
 uint64_t face_key(int dim, const int* fv) {
     if (dim == 1) return (uint64_t)(uint32_t)fv[0];
     uint32_t a = (uint32_t)std::min(fv[0], fv[1]), b = (uint32_t)std::max(fv[0], fv[1]);
@@ -165,7 +220,9 @@ int build(int dim, int nverts, const double* xyz, int ncells, int nvpc, const in
             }
     }
     std::vector<int> perm(ncells), inv(ncells);
-    if (renumber)
+    if (renumber == 2)
+        perm = hilbert_order(dim, ncells, nvpc, cverts, nvc, xyz);
+    else if (renumber)
         perm = rcm_order(ncells, off, adj);
     else
         for (int c = 0; c < ncells; ++c) perm[c] = c;

@@ -147,8 +147,10 @@ __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
 
 // ------------------------------------------------------------------------------ k_prep
 // GRAD: 0 = none (PCM), 1 = Green-Gauss, 2 = least squares
+// 3 waves / SIMD: the hoisted neighbour loads need ~170 VGPRs; measured faster than the
+// 4-wave form with the loads issued inside the face loop.
 template <int MAXF, int GRAD>
-__global__ __launch_bounds__(BLOCK) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
+__global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 const T* __restrict__ X, const T* __restrict__ Ag,
                                                 T* __restrict__ As, T* __restrict__ QF, int use_limiters) {
     if (!ctrl->active) return;
@@ -172,6 +174,27 @@ __global__ __launch_bounds__(BLOCK) void k_prep(DevMesh m, Params P, const Ctrl*
         }
         return;
     } else {
+        // all independent loads of the cell first (indices, then neighbour states and face
+        // geometry through in-bounds clamped indices), so they are in flight together
+        int fk[MAXF], nbk[MAXF], mk[MAXF];
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            fk[k] = m.cface[(size_t)k * ld + c];
+            nbk[k] = m.cnbr[(size_t)k * ld + c];
+            mk[k] = m.cmeta[(size_t)k * ld + c];
+        }
+        T qn[MAXF][NQ], an[MAXF][NA], Nk[MAXF][2], xfk[MAXF][2];
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            const int nbs = nbk[k] >= 0 ? nbk[k] : c;
+            const int fs = fk[k] >= 0 ? fk[k] : 0;
+            load_soa<NQ>(X, ld, nbs, qn[k]);
+            load_soa<NA>(Ag, ld, nbs, an[k]);
+            Nk[k][0] = m.fNx[fs];
+            Nk[k][1] = DIM > 1 ? m.fNy[fs] : (T)0;
+            xfk[k][0] = m.fcx[fs];
+            xfk[k][1] = DIM > 1 ? m.fcy[fs] : (T)0;
+        }
         T lo[NQ], hi[NQ], g[NQ * DIM];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
@@ -184,25 +207,22 @@ __global__ __launch_bounds__(BLOCK) void k_prep(DevMesh m, Params P, const Ctrl*
 
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
-            const int f = m.cface[(size_t)k * ld + c];
+            const int f = fk[k];
             if (f < 0) continue;
-            const int nb = m.cnbr[(size_t)k * ld + c];
-            const T N[2] = {m.fNx[f], DIM > 1 ? m.fNy[f] : (T)0};
+            const int nb = nbk[k];
+            const T* N = Nk[k];
             if (nb >= 0) {
-                T qn[NQ], an[NA];
-                load_soa<NQ>(X, ld, nb, qn);
-                load_soa<NA>(Ag, ld, nb, an);
-                auto rn = M::update_variables(qn, an, P.p, 0.0);
+                auto rn = M::update_variables(qn[k], an[k], P.p, 0.0);
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
                     if (rn[i] < lo[i]) lo[i] = rn[i];
                     if (rn[i] > hi[i]) hi[i] = rn[i];
                 }
                 if constexpr (GRAD == 1) {
-                    const bool right = (m.cmeta[(size_t)k * ld + c] >> 2) & 1;
+                    const bool right = (mk[k] >> 2) & 1;
 #pragma unroll
                     for (int i = 0; i < NQ; ++i) {
-                        const T fv = (T)0.5 * (q[i] + qn[i]);
+                        const T fv = (T)0.5 * (q[i] + qn[k][i]);
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) {
                             const T v = fv * N[d];
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) void k_prep(DevMesh m, Params P, const Ctrl*
                 } else {
 #pragma unroll
                     for (int i = 0; i < NQ; ++i) {
-                        const T du = qn[i] - q[i];
+                        const T du = qn[k][i] - q[i];
 #pragma unroll
                         for (int d = 0; d < DIM; ++d)
                             g[i * DIM + d] += m.lsqw[((size_t)k * DIM + d) * ld + c] * du;
@@ -225,7 +245,7 @@ __global__ __launch_bounds__(BLOCK) void k_prep(DevMesh m, Params P, const Ctrl*
                 const int bc = m.fbc[f];
                 if (bc >= 0) {
                     const T nh[3] = {m.fnhx[f], DIM > 1 ? m.fnhy[f] : (T)0, 0};
-                    const T xf[3] = {m.fcx[f], DIM > 1 ? m.fcy[f] : (T)0, 0};
+                    const T xf[3] = {xfk[k][0], xfk[k][1], 0};
                     auto qb = M::boundary_conditions(bc, q, nullptr, P.p, nh, xf, 0.0, 0.0);
 #pragma unroll
                     for (int i = 0; i < NQ; ++i)
@@ -244,9 +264,8 @@ __global__ __launch_bounds__(BLOCK) void k_prep(DevMesh m, Params P, const Ctrl*
         // limited linear reconstruction at every face of the cell (Reconstruction.hpp:203-243)
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
-            const int f = m.cface[(size_t)k * ld + c];
-            if (f < 0) continue;
-            const T dx[2] = {m.fcx[f] - xc[0], DIM > 1 ? m.fcy[f] - xc[1] : (T)0};
+            if (fk[k] < 0) continue;
+            const T dx[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
             T delta[NQ], fin[NQ];
 #pragma unroll
             for (int i = 0; i < NQ; ++i) {
@@ -427,17 +446,16 @@ __device__ __forceinline__ bool gepp_reg(T* A, T* b, T* x) {
         if (mv < 1e-14) return false;
 #pragma unroll
         for (int i = k + 1; i < NQ; ++i) {
-            if (i == mr) {
+            const bool sw = (i == mr);
 #pragma unroll
-                for (int j = k; j < NQ; ++j) {
-                    const T t = A[k * NQ + j];
-                    A[k * NQ + j] = A[i * NQ + j];
-                    A[i * NQ + j] = t;
-                }
-                const T t = b[k];
-                b[k] = b[i];
-                b[i] = t;
+            for (int j = k; j < NQ; ++j) {
+                const T u = A[k * NQ + j], v = A[i * NQ + j];
+                A[k * NQ + j] = sw ? v : u;
+                A[i * NQ + j] = sw ? u : v;
             }
+            const T u = b[k], v = b[i];
+            b[k] = sw ? v : u;
+            b[i] = sw ? u : v;
         }
 #pragma unroll
         for (int i = k + 1; i < NQ; ++i) {
@@ -633,6 +651,20 @@ __global__ void k_soa_to_aos(const T* __restrict__ soa, T* __restrict__ aos, int
     if (c >= n) return;
     for (int i = 0; i < nc; ++i) aos[(size_t)c * nc + i] = soa[(size_t)i * ld + c];
 }
+// STREAM-style copy used to measure the achievable HBM bandwidth and to calibrate the
+// FETCH_SIZE / WRITE_SIZE counters against a kernel of known traffic.
+__global__ __launch_bounds__(1024) void k_stream_copy(const double* __restrict__ a,
+                                                      double* __restrict__ b, size_t n2) {
+    // streaming (non-temporal) 2 x 8 bytes per element, coalesced across the workgroup
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
+        const double x = __builtin_nontemporal_load(a + 2 * i);
+        const double y = __builtin_nontemporal_load(a + 2 * i + 1);
+        __builtin_nontemporal_store(x, b + 2 * i);
+        __builtin_nontemporal_store(y, b + 2 * i + 1);
+    }
+}
+
 __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -1244,6 +1276,40 @@ int zdm_enforce_constraints(zdm_solver* s) {
     post(s, 4);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int zdm_stream_copy(size_t bytes, int iters, double* gbs, double* avg_ms) {
+    if (!gbs || iters < 1 || bytes < 4096) return fail("zdm_stream_copy: bad arguments");
+    const size_t n = bytes / sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n * sizeof(double2)));
+    if (hipMalloc(&b, n * sizeof(double2)) != hipSuccess) {
+        (void)hipFree(a);
+        return fail("hipMalloc failed in zdm_stream_copy");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned grid = 65536;
+    (void)hipMemset(a, 0, n * sizeof(double2));
+    const double* ad = reinterpret_cast<const double*>(a);
+    double* bd = reinterpret_cast<double*>(b);
+    k_stream_copy<<<grid, 1024>>>(ad, bd, n);
+    (void)hipEventRecord(e0, 0);
+    for (int i = 0; i < iters; ++i) k_stream_copy<<<grid, 1024>>>(ad, bd, n);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(std::string("zdm_stream_copy: ") + hipGetErrorString(e));
+    const double per = ms / iters;
+    *gbs = 2.0 * (double)(n * sizeof(double2)) / (per * 1e-3) / 1e9;
+    if (avg_ms) *avg_ms = per;
     return 0;
 }
 
