@@ -10,14 +10,14 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 MODELS_DIR = os.path.join(ROOT, "zoomy-dmplex_amd", "models")
-MODELS = ["sme0_1d", "sme0", "sme1", "sme2"]
+MODELS = ["sme0_1d", "sme0", "sme1", "sme2", "vam"]
 
 
 def build(model):
     out = os.path.join(HERE, "liboracle_%s.so" % model)
     cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",
            "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(MODELS_DIR, model),
-           os.path.join(HERE, "oracle.cpp"), "-o", out + ".tmp"]
+           "-I" + HERE, os.path.join(HERE, "oracle.cpp"), "-o", out + ".tmp"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed for %s:\n%s" % (model, r.stdout[-4000:]))

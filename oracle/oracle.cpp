@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <vector>
 
 #include <type_traits>
@@ -528,6 +529,13 @@ void ts_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T
 }
 
 }  // namespace
+
+#ifdef ZOOMY_MODEL_IS_VAM
+#include "ChorinOps.H"
+namespace {
+#include "oracle_chorin.inc"
+}  // namespace
+#endif
 
 extern "C" {
 

@@ -70,6 +70,20 @@ class Oracle:
         L.orc_run.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+        if hasattr(L, "orc_vam_info"):
+            L.orc_vam_info.argtypes = [_ip] * 3
+            L.orc_deriv1.argtypes = [M, _dp, ctypes.c_int, _dp]
+            L.orc_fill_ca.argtypes = [M, ctypes.c_int, _dp, _dp, _dp]
+            L.orc_pressure_residual.argtypes = [M, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+            L.orc_assemble_pmat.argtypes = [M, _dp, ctypes.c_double, _dp, _dp, ctypes.c_int,
+                                            _ip, _ip, _dp]
+            L.orc_csr_gmres.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip]
+            L.orc_chorin_run.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                         _ip, _ip]
         self.L = L
         v = [ctypes.c_int() for _ in range(4)]
         L.orc_model_info(*[ctypes.byref(x) for x in v])
@@ -127,6 +141,62 @@ class Oracle:
                            gradient, int(well_balanced), cfl, t_end, min_dt, max_steps, dp(X),
                            dp(A), dp(t), dp(d), int(first))
         return n, t[0], d[0]
+
+
+    # ---- Chorin VAM (liboracle_vam.so only)
+    def vam_info(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        self.L.orc_vam_info(*[ctypes.byref(x) for x in v])
+        return [x.value for x in v]
+
+    def deriv1(self, mesh, f, axis):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros(mesh.ncells)
+        self.L.orc_deriv1(mesh.handle, dp(f), axis, dp(out))
+        return out
+
+    def fill_ca(self, mesh, want_p, X, P, CA):
+        self.L.orc_fill_ca(mesh.handle, int(want_p), dp(X), dp(P), dp(CA))
+
+    def pressure_residual(self, mesh, p, dt, X, P, CA):
+        out = np.zeros_like(P)
+        self.L.orc_pressure_residual(mesh.handle, dp(p), dt, dp(X), dp(P), dp(CA), dp(out))
+        return out
+
+    def assemble_pmat(self, mesh, p, dt, X, CA):
+        """scipy CSR of the assembled pressure matrix (scalar rows c*NP + k)."""
+        import scipy.sparse as sp
+        n = mesh.ncells * 2
+        rp = np.zeros(n + 1, dtype=np.int32)
+        z = np.zeros(1, dtype=np.int32)
+        nnz = self.L.orc_assemble_pmat(mesh.handle, dp(p), dt, dp(X), dp(CA), 0, ip(rp), ip(z),
+                                       dp(np.zeros(1)))
+        ci = np.zeros(nnz, dtype=np.int32)
+        v = np.zeros(nnz)
+        self.L.orc_assemble_pmat(mesh.handle, dp(p), dt, dp(X), dp(CA), nnz, ip(rp), ip(ci), dp(v))
+        return sp.csr_matrix((v, ci, rp), shape=(n, n))
+
+    def csr_gmres(self, A, b, restart=30, max_it=400, rtol=1e-10, atol=1e-50, x0=None):
+        rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        ci = np.ascontiguousarray(A.indices, dtype=np.int32)
+        v = np.ascontiguousarray(A.data, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(A.shape[0]) if x0 is None else np.array(x0, dtype=np.float64)
+        its = ctypes.c_int()
+        reason = self.L.orc_csr_gmres(A.shape[0], ip(rp), ip(ci), dp(v), dp(b), dp(x), restart,
+                                      max_it, rtol, atol, ctypes.byref(its))
+        return x, reason, its.value
+
+    def chorin_run(self, mesh, face_bc, p, X, A, P, order=1, use_limiters=1, cfl=0.45, t_end=1e9,
+                   min_dt=1e-12, max_steps=1, restart=30, max_it=400, rtol=1e-10, atol=1e-50,
+                   op_mode=0, time=0.0):
+        t = np.array([time])
+        its = ctypes.c_int()
+        reason = ctypes.c_int()
+        n = self.L.orc_chorin_run(mesh.handle, ip(face_bc), dp(p), order, use_limiters, cfl, t_end,
+                                  min_dt, max_steps, restart, max_it, rtol, atol, op_mode, dp(X),
+                                  dp(A), dp(P), dp(t), ctypes.byref(its), ctypes.byref(reason))
+        return n, t[0], its.value, reason.value
 
 
 _oracles = {}

@@ -86,6 +86,23 @@ def c2_state(mesh, n_q=8, seed=2):
     return sme_state(n_q, b, h)
 
 
+def c3_state(mesh, seed=3, amp=0.2, x0=0.3, n_bumps=6, bump_hi=0.1, g=9.81):
+    """C3 (VAM): solitary wave of amplitude `amp` on depth 1 over a seeded bump field
+    (heights U(0, bump_hi)); state [b, h, q_x_0, q_x_1, q_y_0, q_y_1, r_0, r_1]."""
+    rng = SplitMix64(seed)
+    xy = mesh.cell_centroid[:, :2]
+    b = gaussian_bumps(xy, rng, n_bumps, bump_hi)
+    kappa = math.sqrt(3.0 * amp / (4.0 * (1.0 + amp)))
+    eta = amp / np.cosh(kappa * (xy[:, 0] - x0) / 0.05) ** 2
+    h = np.maximum(1.0 - b + eta, 1e-3)
+    c = math.sqrt(g * (1.0 + amp))
+    X = np.zeros((xy.shape[0], 8))
+    X[:, 0] = b
+    X[:, 1] = h
+    X[:, 2] = c * eta
+    return X
+
+
 def ensemble_member(i, n_q=6, base_seed=4000, cell_lo=1.0e4, cell_hi=2.0e5):
     """C4 member i: (grid n, jitter seed, state builder)."""
     rng = SplitMix64(base_seed + i)

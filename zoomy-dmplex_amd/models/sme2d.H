@@ -7,10 +7,13 @@
 // with the same signatures as the kernel-pointer typedefs of VirtualSolver.hpp:40-51, so the
 // solver's kernels are instantiated on it exactly as on a generated header.
 //
-// State (n_dof_q = 2L + 4):  [b, h, h*u_0 .. h*u_L, h*v_0 .. h*v_L]
-//   u_0 / v_0 = depth-mean velocity, u_i / v_i (i >= 1) = Legendre moments alpha_i / beta_i.
+// State (n_dof_q = 2L + 4 + NTR):  [b, h, h*u_0 .. h*u_L, h*v_0 .. h*v_L, h*r_1 .. h*r_NTR]
+//   u_0 / v_0 = depth-mean velocity, u_i / v_i (i >= 1) = Legendre moments alpha_i / beta_i,
+//   r_k = NTR depth-integrated quantities advected with the mean velocity (VAM predictor:
+//   the vertical-velocity moments).
 // Aux (n_dof_qaux = 1):      [hinv], the Kurganov-Petrova regularised 1/h (Model.H:220 form).
-// Parameters:                [g, nu, lambda] = gravity, kinematic viscosity, slip length.
+// Parameters:                g = p[0], nu = p[PNU], lambda = p[PLAM] (gravity, kinematic
+//                            viscosity, slip length); SME default layout [g, nu, lambda].
 //
 // Fluxes (Kowalski & Torrilhon 2019), normal component with un = u0.n, a_j = alpha_j.n:
 //   h      : h un
@@ -51,14 +54,16 @@ struct SimpleArray {
 #define PORTABLE_FN
 #endif
 
-template <typename T, int L>
+template <typename T, int L, int NTR = 0, int PNU = 1, int PLAM = 2, int NPAR = 3>
 struct SmeModel2D {
     using K = SmeCoeffs<L>;
     static constexpr int level = L;
     static constexpr int NM = L + 1;                 // moments per direction incl. the mean
-    static constexpr int n_dof_q = 2 * NM + 2;
+    static constexpr int n_tracers = NTR;
+    static constexpr int IT = 2 + 2 * NM;            // first tracer slot
+    static constexpr int n_dof_q = 2 * NM + 2 + NTR;
     static constexpr int n_dof_qaux = 1;
-    static constexpr int n_parameters = 3;
+    static constexpr int n_parameters = NPAR;
     static constexpr int dimension = 2;
     static constexpr double dt_max = 5.0;
     static constexpr int n_dof_gradQ = 0;
@@ -85,8 +90,8 @@ struct SmeModel2D {
         SimpleArray<T, n_dof_q> res;
         for (int i = 0; i < n_dof_q; ++i) res[i] = 0;
         const T hinv = Qaux[0];
-        const T k_slip = p[1] / p[2];
-        const T k_visc = p[1] * hinv * hinv;
+        const T k_slip = p[PNU] / p[PLAM];
+        const T k_visc = p[PNU] * hinv * hinv;
         T mu = 0, mv = 0;
         for (int j = 0; j < NM; ++j) { mu += Q[IU + j]; mv += Q[IV + j]; }
         const T ub = mu * hinv, vb = mv * hinv;
@@ -110,8 +115,8 @@ struct SmeModel2D {
         for (int i = 0; i < n_dof_q * n_dof_q; ++i) J[i] = 0;
         const T hinv = Qaux[0];
         const T hinv2 = hinv * hinv;
-        const T k_slip = p[1] / p[2];
-        const T k_visc = p[1] * hinv2;
+        const T k_slip = p[PNU] / p[PLAM];
+        const T k_visc = p[PNU] * hinv2;
         T mu = 0, mv = 0;
         for (int j = 0; j < NM; ++j) { mu += Q[IU + j]; mv += Q[IV + j]; }
         for (int i = 0; i < NM; ++i) {
@@ -122,8 +127,8 @@ struct SmeModel2D {
                 sv += (T)K::C[i * NM + j] * Q[IV + j];
             }
             // d/dh: u_b = m hinv -> -m hinv^2 ; nu hinv^2 s -> -2 nu hinv^3 s
-            J[(IU + i) * n_dof_q + 1] = w * (k_slip * mu * hinv2 + (T)2 * p[1] * hinv2 * hinv * su);
-            J[(IV + i) * n_dof_q + 1] = w * (k_slip * mv * hinv2 + (T)2 * p[1] * hinv2 * hinv * sv);
+            J[(IU + i) * n_dof_q + 1] = w * (k_slip * mu * hinv2 + (T)2 * p[PNU] * hinv2 * hinv * su);
+            J[(IV + i) * n_dof_q + 1] = w * (k_slip * mv * hinv2 + (T)2 * p[PNU] * hinv2 * hinv * sv);
             for (int j = 0; j < NM; ++j) {
                 const T cij = (j >= 1) ? (T)K::C[i * NM + j] : (T)0;
                 J[(IU + i) * n_dof_q + IU + j] = -w * (k_slip * hinv + k_visc * cij);
@@ -173,6 +178,7 @@ struct SmeModel2D {
                 res[IU + k] = Q[IU + k] - (T)2 * qn * n[0];
                 res[IV + k] = Q[IV + k] - (T)2 * qn * n[1];
             }
+            for (int k = 0; k < NTR; ++k) res[IT + k] = Q[IT + k];
             return res;
         }
         for (int i = 0; i < n_dof_q; ++i) res[i] = 0;
@@ -222,9 +228,10 @@ struct SmeModel2D {
     }
 };
 
-template <typename T, int L>
+template <typename T, int L, int NTR = 0, int PNU = 1, int PLAM = 2, int NPAR = 3>
 struct SmeNumerics2D {
-    using M = SmeModel2D<T, L>;
+    using M = SmeModel2D<T, L, NTR, PNU, PLAM, NPAR>;
+    static constexpr int IT = M::IT;
     using K = SmeCoeffs<L>;
     static constexpr int n_dof_q = M::n_dof_q;
     static constexpr int NM = M::NM;
@@ -235,9 +242,9 @@ struct SmeNumerics2D {
     // Hydrostatically reconstructed state: h* = max(0, h + b - max(bL, bR)), momenta scaled by
     // h*/h; velocities of the reconstructed state through the KP inverse of h*.
     struct HR {
-        T h;            // h*
-        T m[2 * NM];    // reconstructed momenta  (x moments then y moments)
-        T u[2 * NM];    // velocities             (x moments then y moments)
+        T h;                    // h*
+        T m[2 * NM + NTR];      // reconstructed momenta (x moments, y moments) and tracers
+        T u[2 * NM];            // velocities            (x moments then y moments)
     };
 
     PORTABLE_FN static inline void reconstruct(const T* Q, T bstar, HR& s) {
@@ -248,6 +255,7 @@ struct SmeNumerics2D {
             s.m[k] = Q[IU + k] * r;
             s.u[k] = s.m[k] * hi;
         }
+        for (int k = 0; k < NTR; ++k) s.m[2 * NM + k] = Q[IT + k] * r;
     }
 
     // normal flux F(q*) . n of a reconstructed state (b slot is zero)
@@ -283,6 +291,8 @@ struct SmeNumerics2D {
             F[IU + i] = gx;
             F[IV + i] = gy;
         }
+        // tracers ride the mean velocity: h r_k un
+        for (int k = 0; k < NTR; ++k) F[IT + k] = m[2 * NM + k] * un;
     }
 
     PORTABLE_FN static inline T speed(const HR& s, const T* p, const T* n) {
@@ -320,7 +330,7 @@ struct SmeNumerics2D {
         T dq[n_dof_q];
         dq[0] = 0;
         dq[1] = sR.h - sL.h;
-        for (int k = 0; k < 2 * NM; ++k) dq[IU + k] = sR.m[k] - sL.m[k];
+        for (int k = 0; k < 2 * NM + NTR; ++k) dq[IU + k] = sR.m[k] - sL.m[k];
         T nc[n_dof_q];
         for (int i = 0; i < n_dof_q; ++i) nc[i] = 0;
         if (NM > 1) {
