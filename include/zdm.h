@@ -103,6 +103,57 @@ int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches);
  * over `iters` launches: the achievable-peak reference and the traffic-counter calibration. */
 int zdm_stream_copy(size_t bytes, int iters, double* gbs, double* avg_ms);
 
+/* ---- Chorin pressure split for VAM models (ChorinVAMSolver, ChorinVAM.hpp:86-129).
+ * Available in libraries built for a model whose Model.H defines ZOOMY_MODEL_IS_VAM and
+ * whose directory provides ChorinOps.H; elsewhere these return an error, as the reference's
+ * stub ChorinVAMSolver::Run does (ChorinVAM.hpp:26-37).  The preconditioner is always
+ * point-block Jacobi (-pc_type pbjacobi) on the device-assembled 2x2-block Pmat. */
+typedef struct zdm_krylov_options {
+    int restart;     /* KSPGMRESSetRestart (ChorinVAM.hpp:403; default 40) */
+    int max_it;      /* KSPSetTolerances max_it (ChorinVAM.hpp:449; default 400) */
+    double rtol;     /* default 1e-9 */
+    double atol;     /* default 1e-12 */
+    double dtol;     /* PETSc default 1e5 */
+    int op_mode;     /* 0 = matrix-free residual(P) - R0 (the MatShell Aop, ChorinVAM.hpp:577-584);
+                        1 = the assembled Pmat as the operator */
+} zdm_krylov_options;
+
+/* Krylov settings of the pressure solve (SetupChorin + KSPSetFromOptions,
+ * ChorinVAM.hpp:386-459); NULL restores the defaults.  The stencils and the block pattern are
+ * built by zdm_solver_create. */
+int zdm_chorin_setup(zdm_solver* s, const zdm_krylov_options* opts);
+
+/* n Chorin steps of ChorinVAMSolver::Run's loop (ChorinVAM.hpp:102-125) on the resident
+ * state, after zdm_initialize: dt = max(CFL, min_dt) clamped to t_end, predictor
+ * (FormRHS(0) + forward Euler + source + constraints), frozen derivative aux, Pmat + PBJ,
+ * GMRES on A P = -R0 from P = 0, corrector, constraints.  Reports the steps taken, the
+ * total GMRES iterations and the last KSPConvergedReason (2 rtol, 3 atol, -3 max_it,
+ * -4 dtol, -9 NaN).  Synchronises once per GMRES iteration. */
+int zdm_chorin_step(zdm_solver* s, int n, int* steps_done, int* its_total, int* last_reason);
+
+/* The last solved pressure, NP = 2 per cell (the dmP Vec layout). */
+int zdm_chorin_pressure(zdm_solver* s, double* P);
+
+/* Parts of one step on the resident state, exposed for parity tests and callers that drive
+ * the split themselves:
+ *   zdm_vam_frozen_aux: ComputeFrozenAux (FillCA(false), ChorinVAM.hpp:521-546); CA (may be
+ *     NULL) receives all NCA slots per cell (pressure slots as last computed);
+ *   zdm_vam_residual: PressureResidual (ChorinVAM.hpp:551-576) of P (NP per cell) with the
+ *     frozen aux of the last zdm_vam_frozen_aux;
+ *   zdm_vam_pmat: AssemblePmat (ChorinVAM.hpp:242-303) as 2x2-block CSR (row pointers
+ *     ncells+1, block columns, 4 values per block row-major); returns the block count, and
+ *     only that when cap is smaller. */
+int zdm_vam_frozen_aux(zdm_solver* s, double* CA);
+int zdm_vam_residual(zdm_solver* s, double dt, const double* P, double* R);
+int zdm_vam_pmat(zdm_solver* s, double dt, int cap, int* rowptr, int* col, double* val);
+
+/* GMRES(restart) + point-block Jacobi on a 2x2-block CSR system A x = b (host arrays; x holds
+ * the initial guess): KSPGMRES with classical Gram-Schmidt, left preconditioning and the
+ * preconditioned-residual test of KSPConvergedDefault, as the pressure solve configures it
+ * (ChorinVAM.hpp:405-451).  Exported by every model library. */
+int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val, const double* b,
+                   double* x, const zdm_krylov_options* opts, int* its, int* reason);
+
 const char* zdm_last_error(void);
 
 #ifdef __cplusplus

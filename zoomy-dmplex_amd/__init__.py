@@ -253,7 +253,19 @@ SOLVER_SYMBOLS = [
     "zdm_transport_rhs", "zdm_source_step", "zdm_enforce_constraints", "zdm_cfl_dt",
     "zdm_state_upload_device", "zdm_state_download_device", "zdm_enable_timing",
     "zdm_kernel_time", "zdm_stream_copy", "zdm_last_error",
+    "zdm_chorin_setup", "zdm_chorin_step", "zdm_chorin_pressure", "zdm_vam_frozen_aux",
+    "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres",
 ]
+
+
+class KrylovOptions(ctypes.Structure):
+    """zdm_krylov_options: GMRES(restart) + point-block Jacobi of the pressure solve.
+    Defaults are the reference's KSP settings (ChorinVAM.hpp:403, 449)."""
+    _fields_ = [("restart", ctypes.c_int), ("max_it", ctypes.c_int), ("rtol", ctypes.c_double),
+                ("atol", ctypes.c_double), ("dtol", ctypes.c_double), ("op_mode", ctypes.c_int)]
+
+    def __init__(self, restart=40, max_it=400, rtol=1e-9, atol=1e-12, dtol=1e5, op_mode=0):
+        super().__init__(restart, max_it, rtol, atol, dtol, op_mode)
 
 
 def model_lib(model):
@@ -286,6 +298,15 @@ def model_lib(model):
     lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
     lib.zdm_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _c_dbl_p, _c_dbl_p]
     lib.zdm_last_error.restype = ctypes.c_char_p
+    K = ctypes.POINTER(KrylovOptions)
+    lib.zdm_chorin_setup.argtypes = [V, K]
+    lib.zdm_chorin_step.argtypes = [V, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
+    lib.zdm_chorin_pressure.argtypes = [V, _c_dbl_p]
+    lib.zdm_vam_frozen_aux.argtypes = [V, _c_dbl_p]
+    lib.zdm_vam_residual.argtypes = [V, ctypes.c_double, _c_dbl_p, _c_dbl_p]
+    lib.zdm_vam_pmat.argtypes = [V, ctypes.c_double, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p]
+    lib.zdm_bsr2_gmres.argtypes = [ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                   K, _c_int_p, _c_int_p]
     _model_libs[model] = lib
     return lib
 
@@ -443,6 +464,74 @@ class Solver:
         ms, n = ctypes.c_double(), ctypes.c_int()
         self._check(self.lib.zdm_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    # ---- Chorin pressure split (VAM models, ChorinVAMSolver, ChorinVAM.hpp:86-129)
+    def chorin_setup(self, krylov=None, **kw):
+        """Krylov settings of the pressure solve (KrylovOptions or its fields by name)."""
+        o = krylov if krylov is not None else KrylovOptions(**kw)
+        self._check(self.lib.zdm_chorin_setup(self._h, ctypes.byref(o)))
+        self.krylov = o
+
+    def chorin_step(self, n=1):
+        """n Chorin steps; returns (steps taken, total GMRES iterations, last KSP reason)."""
+        st, its, rs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.zdm_chorin_step(self._h, int(n), ctypes.byref(st), ctypes.byref(its),
+                                             ctypes.byref(rs)))
+        return st.value, its.value, rs.value
+
+    def chorin_pressure(self):
+        P = np.empty((self.mesh.ncells, 2))
+        self._check(self.lib.zdm_chorin_pressure(self._h, _dp(P)))
+        return P
+
+    def vam_frozen_aux(self):
+        CA = np.empty((self.mesh.ncells, 20))
+        self._check(self.lib.zdm_vam_frozen_aux(self._h, _dp(CA)))
+        return CA
+
+    def vam_residual(self, dt, P):
+        P = np.ascontiguousarray(P, dtype=np.float64).reshape(self.mesh.ncells, 2)
+        R = np.empty_like(P)
+        self._check(self.lib.zdm_vam_residual(self._h, float(dt), _dp(P), _dp(R)))
+        return R
+
+    def vam_pmat(self, dt):
+        """Assembled pressure matrix as 2x2-block CSR: (row pointers, block columns,
+        values (nnzb, 2, 2))."""
+        z = np.zeros(1, dtype=np.int32)
+        nnzb = self.lib.zdm_vam_pmat(self._h, float(dt), 0, None, None, None)
+        if nnzb < 0:
+            raise ZdmError(self.lib.zdm_last_error().decode())
+        rp = np.empty(self.mesh.ncells + 1, dtype=np.int32)
+        col = np.empty(nnzb, dtype=np.int32)
+        val = np.empty((nnzb, 2, 2))
+        r = self.lib.zdm_vam_pmat(self._h, float(dt), nnzb, rp.ctypes.data_as(_c_int_p),
+                                  col.ctypes.data_as(_c_int_p), _dp(val))
+        if r != nnzb:
+            raise ZdmError(self.lib.zdm_last_error().decode())
+        del z
+        return rp, col, val
+
+
+def bsr2_gmres(rowptr, col, val, b, x0=None, krylov=None, model="sme2", **kw):
+    """GMRES + point-block Jacobi on a 2x2-block CSR system on the device; returns
+    (x, iterations, KSPConvergedReason)."""
+    lib = model_lib(model)
+    o = krylov if krylov is not None else KrylovOptions(**kw)
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    nb = rowptr.shape[0] - 1
+    if val.size != 4 * col.size or b.size != 2 * nb:
+        raise ZdmError("bsr2_gmres: inconsistent sizes")
+    x = np.zeros(2 * nb) if x0 is None else np.array(x0, dtype=np.float64)
+    its, rs = ctypes.c_int(), ctypes.c_int()
+    if lib.zdm_bsr2_gmres(nb, rowptr.ctypes.data_as(_c_int_p), col.ctypes.data_as(_c_int_p),
+                          _dp(val), _dp(b), _dp(x), ctypes.byref(o), ctypes.byref(its),
+                          ctypes.byref(rs)) != 0:
+        raise ZdmError(lib.zdm_last_error().decode())
+    return x, its.value, rs.value
 
 
 def stream_copy_gbs(model="sme2", nbytes=1 << 30, iters=20):

@@ -32,6 +32,9 @@
 
 #pragma clang force_cuda_host_device begin
 #include "Numerics.H"
+#ifdef ZOOMY_MODEL_IS_VAM
+#include "ChorinOps.H"
+#endif
 #pragma clang force_cuda_host_device end
 
 #ifndef ZDM_MODEL_NAME
@@ -568,7 +571,8 @@ __device__ __forceinline__ T block_min(T v) {
 }
 
 // POST: 0 = CFL only, 1 = constraints + CFL, 2 = constraints + source + constraints + CFL,
-//       3 = source only (no CFL), 4 = constraints only (no CFL)
+//       3 = source only (no CFL), 4 = constraints only (no CFL),
+//       5 = source with the step's dt + constraints (no CFL; the Chorin predictor)
 template <int POST>
 __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl* __restrict__ ctrl, T* X,
                                                 T* A, T* __restrict__ partials, T dt_override) {
@@ -581,11 +585,11 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
         load_soa<NQ>(X, ld, c, q);
         load_soa<NA>(A, ld, c, a);
         if constexpr (POST == 1 || POST == 2 || POST == 4) update_state_cell(P, q, a);
-        if constexpr (POST == 2 || POST == 3) {
-            const T dt = POST == 2 ? ctrl->dt : dt_override;
+        if constexpr (POST == 2 || POST == 3 || POST == 5) {
+            const T dt = POST == 3 ? dt_override : ctrl->dt;
             newton_cell(P, dt, q, a);
         }
-        if constexpr (POST == 2) update_state_cell(P, q, a);
+        if constexpr (POST == 2 || POST == 5) update_state_cell(P, q, a);
         if constexpr (POST != 0) {
             store_soa<NQ>(X, ld, c, q);
             store_soa<NA>(A, ld, c, a);
@@ -598,7 +602,8 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
     }
 }
 
-// what: 0 = standalone ComputeTimeStep (-> dt_query), 1 = first dt, 2 = end of a step
+// what: 0 = standalone ComputeTimeStep (-> dt_query), 1 = first dt, 2 = end of a step,
+//       3 = end of a Chorin step (next dt = max(CFL, min_dt), ChorinVAM.hpp:103)
 __global__ __launch_bounds__(BLOCK) void k_finalize(Ctrl* ctrl, const T* __restrict__ partials, int np,
                                                     int what, int order, int s, T cfl, T min_dt,
                                                     T t_end, T dt_max) {
@@ -619,7 +624,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(Ctrl* ctrl, const T* __restr
     } else {
         ctrl->t = ctrl->t + ctrl->dt;
         ctrl->steps = ctrl->steps + 1;
-        ctrl->dt_cfl = dtc;
+        ctrl->dt_cfl = (what == 3 && dtc < min_dt) ? min_dt : dtc;
     }
     ctrl->finite = fin;
     // prepare the next step (MUSCLSolver.hpp:140-149)
@@ -670,6 +675,12 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
     if (i < n) a[i] = v;
 }
 
+#include "krylov.inc"
+#ifdef ZOOMY_MODEL_IS_VAM
+#include "chorin.inc"
+#endif
+struct ChorinHost;
+
 }  // namespace
 
 // ================================================================================ host side
@@ -690,6 +701,7 @@ struct zdm_solver {
     size_t evused;
     double kt_ms[3];
     int kt_n[3];
+    ChorinHost* ch;                   // pressure split state (VAM models)
 };
 
 namespace {
@@ -777,6 +789,8 @@ void flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* 
 void post(zdm_solver* s, int mode, T dt_override = 0) {
     dim3 g(s->nblocks), b(BLOCK);
     switch (mode) {
+        case 5: k_post<5><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
+        case 6: k_post<4><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
         case 0: k_post<0><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
         case 1: k_post<1><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
         case 2: k_post<2><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
@@ -790,6 +804,13 @@ void finalize(zdm_solver* s, int what) {
                                           s->opt.ssp_stages, s->opt.cfl, s->opt.min_dt, s->opt.t_end,
                                           (T)M::dt_max);
 }
+
+#ifdef ZOOMY_MODEL_IS_VAM
+#include "chorin_host.inc"
+#else
+struct ChorinHost {};
+void chorin_free(ChorinHost*) {}
+#endif
 
 int set_ctrl_rhs_time(zdm_solver* s, T t) {
     Ctrl c;
@@ -1040,6 +1061,13 @@ int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* pa
     Ctrl c0;
     std::memset(&c0, 0, sizeof(c0));
     hipMemcpy(s->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice);
+#ifdef ZOOMY_MODEL_IS_VAM
+    if (chorin_build(s, mesh, krylov_defaults())) {
+        const std::string msg = g_err;
+        zdm_solver_destroy(s);
+        return fail(msg);
+    }
+#endif
     *out = s;
     return 0;
 }
@@ -1048,6 +1076,7 @@ int zdm_solver_destroy(zdm_solver* s) {
     if (!s) return 0;
     hipStreamSynchronize(s->stream);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
+    chorin_free(s->ch);
     hipFree(s->arena);
     hipStreamDestroy(s->stream);
     delete s;
@@ -1333,5 +1362,188 @@ int zdm_cfl_dt(zdm_solver* s, double* dt) {
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
 }
+
+// ------------------------------------------------------------------ Chorin pressure split
+
+int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val, const double* b,
+                   double* x, const zdm_krylov_options* opts, int* its, int* reason) {
+    if (nb < 1 || !rowptr || !col || !val || !b || !x) return fail("zdm_bsr2_gmres: bad arguments");
+    zdm_krylov_options o = opts ? *opts : krylov_defaults();
+    if (check_krylov(o)) return -1;
+    const int nnzb = rowptr[nb];
+    if (nnzb < 0 || rowptr[0] != 0) return fail("zdm_bsr2_gmres: bad row pointers");
+    for (int c = 0; c < nb; ++c)
+        if (rowptr[c + 1] < rowptr[c]) return fail("zdm_bsr2_gmres: row pointers must not decrease");
+    for (int k = 0; k < nnzb; ++k)
+        if (col[k] < 0 || col[k] >= nb) return fail("zdm_bsr2_gmres: block column out of range");
+    const size_t n = 2 * (size_t)nb;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t r = off;
+        off = align_up(off + bytes, 256);
+        return r;
+    };
+    const size_t oR = take(sizeof(int) * (nb + 1)), oC = take(sizeof(int) * std::max(nnzb, 1)),
+                 oV = take(sizeof(T) * 4 * std::max(nnzb, 1)), oB = take(sizeof(T) * n), oX = take(sizeof(T) * n),
+                 oD = take(sizeof(T) * 4 * (size_t)nb);
+    char* mem = nullptr;
+    char* kmem = nullptr;
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc = 0;
+    if (hipMalloc(&mem, off) != hipSuccess || hipMalloc(&kmem, gmres_bytes(o.restart, n)) != hipSuccess) {
+        rc = fail("zdm_bsr2_gmres: hipMalloc failed");
+    } else {
+        hipError_t e = hipMemcpyAsync(mem + oR, rowptr, sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && nnzb) e = hipMemcpyAsync(mem + oC, col, sizeof(int) * nnzb, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && nnzb) e = hipMemcpyAsync(mem + oV, val, sizeof(T) * 4 * nnzb, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(mem + oB, b, sizeof(T) * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(mem + oX, x, sizeof(T) * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemsetAsync(kmem, 0, gmres_bytes(o.restart, n), st);
+        if (e != hipSuccess) {
+            rc = fail(std::string("zdm_bsr2_gmres: upload failed: ") + hipGetErrorString(e));
+        } else {
+            Bsr2 A;
+            A.nb = nb;
+            A.rowptr = (const int*)(mem + oR);
+            A.col = (const int*)(mem + oC);
+            A.val = (const T*)(mem + oV);
+            T* Dinv = (T*)(mem + oD);
+            const unsigned cg = (unsigned)((nb + 255) / 256);
+            k_pbj_from_bsr<<<cg, 256, 0, st>>>(A, Dinv);
+            Gmres g;
+            gmres_bind(g, kmem, o.restart, n);
+            g.max_it = o.max_it;
+            g.rtol = o.rtol;
+            g.atol = o.atol;
+            g.dtol = o.dtol;
+            auto op = [&](const T* xx, T* y) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 0); };
+            auto op_pc = [&](const T* xx, T* y) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 1); };
+            auto pc = [&](const T* bb, const T* t, T* z) { k_pbj_residual<<<cg, 256, 0, st>>>(nb, Dinv, bb, t, z); };
+            const int r = gmres_solve(g, st, op_pc, op, pc, (const T*)(mem + oB), (T*)(mem + oX));
+            if (r == -100 || hipGetLastError() != hipSuccess) {
+                rc = fail("zdm_bsr2_gmres: device error in the solve");
+            } else if (hipMemcpyAsync(x, mem + oX, sizeof(T) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                       hipStreamSynchronize(st) != hipSuccess) {
+                rc = fail("zdm_bsr2_gmres: download failed");
+            } else {
+                if (its) *its = g.last_its;
+                if (reason) *reason = g.last_reason;
+            }
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    if (mem) (void)hipFree(mem);
+    if (kmem) (void)hipFree(kmem);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+#ifdef ZOOMY_MODEL_IS_VAM
+
+int zdm_chorin_setup(zdm_solver* s, const zdm_krylov_options* opts) {
+    if (!s || !s->ch) return fail("null solver");
+    const zdm_krylov_options o = opts ? *opts : krylov_defaults();
+    if (check_krylov(o)) return -1;
+    ChorinHost* ch = s->ch;
+    if (o.restart != ch->g.m) {
+        const size_t n = 2 * (size_t)s->ncells;
+        char* km = nullptr;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMalloc(&km, gmres_bytes(o.restart, n)));
+        if (hipMemsetAsync(km, 0, gmres_bytes(o.restart, n), s->stream) != hipSuccess) {
+            (void)hipFree(km);
+            return fail("zdm_chorin_setup: clearing the GMRES workspace failed");
+        }
+        (void)hipFree(ch->kmem);
+        ch->kmem = km;
+        gmres_bind(ch->g, km, o.restart, n);
+    }
+    ch->opt = o;
+    ch->g.max_it = o.max_it;
+    ch->g.rtol = o.rtol;
+    ch->g.atol = o.atol;
+    ch->g.dtol = o.dtol;
+    return 0;
+}
+
+int zdm_chorin_step(zdm_solver* s, int n, int* steps_done, int* its_total, int* last_reason) {
+    if (!s || !s->ch) return fail("null solver");
+    int done = 0, its = 0, reason = 0;
+    for (int i = 0; i < n; ++i) {
+        const int r = chorin_one_step(s);
+        if (r < 0) return -1;
+        if (r == 1) break;
+        ++done;
+        its += s->ch->last_its;
+        reason = s->ch->last_reason;
+    }
+    if (steps_done) *steps_done = done;
+    if (its_total) *its_total = its;
+    if (last_reason) *last_reason = reason;
+    return 0;
+}
+
+int zdm_chorin_pressure(zdm_solver* s, double* P) {
+    if (!s || !s->ch || !P) return fail("zdm_chorin_pressure: null argument");
+    HIPCHK(hipMemcpyAsync(P, s->ch->P, sizeof(T) * 2 * (size_t)s->ncells, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int zdm_vam_frozen_aux(zdm_solver* s, double* CA) {
+    if (!s || !s->ch) return fail("null solver");
+    vam_frozen(s);
+    HIPCHK(hipGetLastError());
+    if (CA) {
+        std::vector<T> soa((size_t)VNCA * s->ld);
+        HIPCHK(hipMemcpyAsync(soa.data(), s->ch->v.CA, sizeof(T) * soa.size(), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int c = 0; c < s->ncells; ++c)
+            for (int j = 0; j < VNCA; ++j) CA[(size_t)c * VNCA + j] = soa[(size_t)j * s->ld + c];
+    }
+    return 0;
+}
+
+int zdm_vam_residual(zdm_solver* s, double dt, const double* P, double* R) {
+    if (!s || !s->ch || !P || !R) return fail("zdm_vam_residual: null argument");
+    ChorinHost* ch = s->ch;
+    const size_t bytes = sizeof(T) * 2 * (size_t)s->ncells;
+    HIPCHK(hipMemcpyAsync(ch->P, P, bytes, hipMemcpyHostToDevice, s->stream));
+    vam_apply(s, P2_RESIDUAL, dt, ch->P, ch->rhs);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(R, ch->rhs, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int zdm_vam_pmat(zdm_solver* s, double dt, int cap, int* rowptr, int* col, double* val) {
+    if (!s || !s->ch) return fail("null solver");
+    ChorinHost* ch = s->ch;
+    const int nnzb = ch->v.nnzb;
+    if (cap < nnzb || !rowptr || !col || !val) return nnzb;
+    vam_assemble(s, dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(val, ch->v.bval, sizeof(T) * 4 * (size_t)nnzb, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::copy(ch->h_brow.begin(), ch->h_brow.end(), rowptr);
+    std::copy(ch->h_bcol.begin(), ch->h_bcol.end(), col);
+    return nnzb;
+}
+
+#else
+
+static int not_vam() {
+    return fail("solver.method=\"chorin\" needs a VAM model (ChorinOps.H and ZOOMY_MODEL_IS_VAM); "
+                "this library was built for " ZDM_MODEL_NAME);
+}
+int zdm_chorin_setup(zdm_solver*, const zdm_krylov_options*) { return not_vam(); }
+int zdm_chorin_step(zdm_solver*, int, int*, int*, int*) { return not_vam(); }
+int zdm_chorin_pressure(zdm_solver*, double*) { return not_vam(); }
+int zdm_vam_frozen_aux(zdm_solver*, double*) { return not_vam(); }
+int zdm_vam_residual(zdm_solver*, double, const double*, double*) { return not_vam(); }
+int zdm_vam_pmat(zdm_solver*, double, int, int*, int*, double*) { return not_vam(); }
+
+#endif
 
 }  // extern "C"
