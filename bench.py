@@ -9,6 +9,10 @@ source, constraints, CFL reduction) over all cells, with the state resident in H
 
 Prints ONE JSON line with the metric, the HBM roofline of the dominant kernel (flux) from
 HIP events on the solver stream, and a CPU baseline (the oracle, 1 thread, bounded sample).
+
+--workload c3 measures the VAM Chorin pressure split instead: 707 x 707 grid jittered by
++-15 % (999 698 cells), solitary wave of amplitude 0.2 over a seeded bump field, each step =
+predictor + GMRES(30) / point-block Jacobi to rtol 1e-10 + corrector.
 """
 import argparse
 import importlib.util
@@ -77,8 +81,187 @@ def cpu_baseline(zdm, cases, grid=250, steps=20):
     }
 
 
+def c3_cpu_baseline(zdm, cases, grid=60, steps=1, krylov=None):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import oracle  # CPU checker, timed here as the reported baseline
+    orc = oracle("vam")
+    mesh = zdm.Mesh.tri_grid(grid, grid, jitter=0.15, seed=3)
+    X = cases.c3_state(mesh)
+    A = np.zeros((mesh.ncells, orc.n_aux))
+    fb = zdm.boundary_face_bc(mesh, ["default"], 2)
+    runs, its = [], []
+    for n in (steps, 2 * steps):
+        Xc, Ac, Pc = X.copy(), A.copy(), np.zeros((mesh.ncells, 2))
+        t0 = time.perf_counter()
+        _, _, it, reason = orc.chorin_run(mesh, fb, orc.default_params, Xc, Ac, Pc, order=1,
+                                          max_steps=n, **krylov)
+        runs.append(time.perf_counter() - t0)
+        its.append(it)
+    per_step = (runs[1] - runs[0]) / steps
+    per_it = (runs[1] - runs[0]) / max(its[1] - its[0], 1)
+    return {
+        "value": mesh.ncells / per_step / 1e6,
+        "unit": "Mcell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "mcell_its_per_s": round(mesh.ncells / per_it / 1e6, 3),
+        "sample": "oracle (C++ restatement of ChorinVAMSolver, sequential GMRES, 1 thread) on a "
+                  "%dx%d jittered grid (%d cells), slope of %d vs %d steps, %.1f GMRES its/step"
+                  % (grid, grid, mesh.ncells, steps, 2 * steps, (its[1] - its[0]) / steps),
+    }
+
+
+def run_c3(zdm, cases, args):
+    n = args.grid if args.grid != 1000 else 707
+    mesh = zdm.Mesh.tri_grid(n, n, jitter=0.15, seed=3, renumber=args.order)
+    X = cases.c3_state(mesh)
+    krylov = dict(restart=30, max_it=args.max_it, rtol=1e-10, atol=1e-12, op_mode=args.op_mode)
+    s = zdm.Solver("vam", mesh, order=1, cfl=0.45, t_end=1e9, source=True)
+    s.chorin_setup(**krylov)
+    s.upload(X, None)
+    s.initialize()
+    steps_w, its_w, _ = s.chorin_step(args.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    steps, its, reason = s.chorin_step(args.steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    t_sim, dt, nsteps = s.time()
+    Xg, _ = s.download()
+    if not np.all(np.isfinite(Xg)):
+        raise RuntimeError("non-finite state after the benchmark")
+    ncells = mesh.ncells
+    out = {
+        "metric": "Mcell-steps/s (VAM Chorin, FP64)",
+        "value": round(ncells * steps / elapsed / 1e6, 4),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": steps_w,
+        "ms_per_step": round(elapsed / max(steps, 1) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "VAM level 1 Chorin split, %dx%d grid jittered +-15%% (%d cells), "
+                        "solitary wave a=0.2 over bumps <= 0.1, GMRES(30) + PBJ rtol 1e-10"
+                        % (n, n, ncells),
+            "cells": ncells,
+            "op_mode": ["matrix-free residual", "assembled Pmat"][args.op_mode],
+            "gmres_its_per_step": round(its / max(steps, 1), 2),
+            "last_reason": reason,
+            "max_it": args.max_it,
+            "ms_per_gmres_it": round(elapsed * 1e3 / max(its, 1), 4),
+            "mcell_its_per_s": round(ncells * its / elapsed / 1e6, 3),
+            "sim_time": t_sim,
+        },
+        "roofline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = c3_cpu_baseline(zdm, cases, krylov=dict(krylov, op_mode=args.op_mode))
+    s.close()
+    print(json.dumps(out))
+
+
+def c4_members(zdm, cases, count):
+    meshes, states = [], []
+    for i in range(count):
+        n, mesh_seed, state = cases.ensemble_member(i)
+        m = zdm.Mesh.tri_grid(n, n, jitter=0.15, seed=mesh_seed)
+        meshes.append(m)
+        states.append(state(m))
+    return meshes, states
+
+
+def run_c4(zdm, cases, args):
+    """Batched ensemble vs the same runs one at a time (both replay a graph-captured step)."""
+    count = args.members
+    meshes, states = c4_members(zdm, cases, count)
+    kw = dict(order=2, ssp_stages=2, cfl=0.45, t_end=1e9, source=True, use_limiters=True)
+    e = zdm.Ensemble("sme1", meshes, **kw)
+    ncells = e.mesh.ncells
+    e.upload(e.join(states))
+    e.initialize()
+    e.step_graph(args.warmup)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.step_graph(args.steps)
+    e.synchronize()
+    t_batch = time.perf_counter() - t0
+    Xe, _ = e.download()
+    if not np.all(np.isfinite(Xe)):
+        raise RuntimeError("non-finite ensemble state")
+    dts = [e.member_time(i)[1] for i in range(count)]
+    e.close()
+    # the same runs one at a time: replaying a captured step, and with plain launches
+    t_serial = t_eager = 0.0
+    for m, X in zip(meshes, states):
+        s = zdm.Solver("sme1", m, **kw)
+        s.upload(X)
+        s.initialize()
+        s.step_graph(args.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step_graph(args.steps)
+        s.synchronize()
+        t_serial += time.perf_counter() - t0
+        s.upload(X)
+        s.initialize()
+        s.step(args.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(args.steps)
+        s.synchronize()
+        t_eager += time.perf_counter() - t0
+        s.close()
+    F = e.mesh.nfaces / ncells
+    bpc = step_bytes_per_cell(6, F)
+    value = ncells * args.steps / t_batch / 1e6
+    out = {
+        "metric": "batched vs serial ensemble (SME L1, FP64)",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_batch / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d SME level 1 members, jittered split grids of %d..%d cells (%d in all), "
+                        "MUSCL + Barth-Jespersen, SSP-RK2, CFL 0.45, own dt per member"
+                        % (count, min(m.ncells for m in meshes), max(m.ncells for m in meshes), ncells),
+            "members": count,
+            "cells": ncells,
+            "serial_mcell_steps_per_s": round(ncells * args.steps / t_serial / 1e6, 3),
+            "batched_s": round(t_batch, 4),
+            "serial_s": round(t_serial, 4),
+            "speedup_batched_vs_serial": round(t_serial / t_batch, 3),
+            "serial_eager_s": round(t_eager, 4),
+            "speedup_batched_vs_serial_eager": round(t_eager / t_batch, 3),
+            "dt_range": [min(dts), max(dts)],
+            "step_bytes_per_cell": round(bpc, 1),
+            "step_gbs": round(bpc * ncells * args.steps / t_batch / 1e9, 1),
+        },
+        "roofline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: SME level 2 transport + source (headline); c3: VAM Chorin split; "
+                         "c4: batched vs serial ensemble")
+    ap.add_argument("--members", type=int, default=256, help="c4 ensemble size")
+    ap.add_argument("--op-mode", type=int, default=0, choices=[0, 1],
+                    help="c3 pressure operator: 0 matrix-free residual, 1 assembled Pmat")
+    ap.add_argument("--max-it", type=int, default=400, help="c3 GMRES iteration cap")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -94,6 +277,10 @@ def main():
 
     zdm = load_package()
     import zoomy_dmplex_amd.cases as cases
+    if args.workload == "c3":
+        return run_c3(zdm, cases, args)
+    if args.workload == "c4":
+        return run_c4(zdm, cases, args)
 
     n = args.grid
     mesh = zdm.Mesh.tri_grid(n, n, renumber=args.order)

@@ -56,6 +56,18 @@ int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* pa
                       const zdm_options* opts, zdm_solver** out);
 int zdm_solver_destroy(zdm_solver* s);
 
+/* Ensemble of independent runs in one set of launches (the batching form).  `mesh` is the
+ * disjoint union of the members (zdm_mesh_concat); member i owns cells [cell_offsets[i],
+ * cell_offsets[i+1]) and whole workgroups, and keeps its own time, dt (local CFL over its
+ * own cells), step count and t_end test, exactly as if it ran alone through
+ * zdm_solver_create.  Every other entry point works on the union state; zdm_get_time reports
+ * member 0 and zdm_batch_time any member.  Replaces running the reference's solver once
+ * per member (one DMPlex + TS per run, VirtualSolver.hpp:386-525). */
+int zdm_batch_create(const zdm_mesh* mesh, const int* face_bc, const double* params,
+                     const zdm_options* opts, int nmembers, const int* cell_offsets,
+                     zdm_solver** out);
+int zdm_batch_time(zdm_solver* s, int member, double* time, double* dt, int* steps);
+
 /* Host <-> device state transfer (AoS, n_q / n_aux per cell).  A may be NULL (aux = 0). */
 int zdm_state_upload(zdm_solver* s, const double* X, const double* A);
 int zdm_state_download(zdm_solver* s, double* X, double* A);
@@ -68,6 +80,10 @@ int zdm_initialize(zdm_solver* s);
  * t_end, TSStep (SSP-RK / Euler), constraints, split source, constraints, CFL for the next
  * dt.  Steps past t_end are no-ops.  Asynchronous: no host synchronisation. */
 int zdm_step(zdm_solver* s, int n);
+
+/* As zdm_step, with the launches of one step captured once into a HIP graph and replayed
+ * n times (same kernels and arguments, so the same results as zdm_step). */
+int zdm_step_graph(zdm_solver* s, int n);
 
 /* Run until t_end (or max_steps); synchronises every `check_every` steps. */
 int zdm_run(zdm_solver* s, int max_steps, int check_every, int* steps_done);

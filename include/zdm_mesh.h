@@ -80,6 +80,13 @@ int zdm_mesh_line(int n, double x0, double x1, int renumber, zdm_mesh** out);
 /* Gmsh MSH 2.2 ASCII reader (points, lines, triangles, quadrilaterals). */
 int zdm_mesh_read_msh(const char* path, int renumber, zdm_mesh** out);
 
+/* Disjoint union of n meshes (an ensemble): member i's cells are [cell_offsets[i],
+ * cell_offsets[i+1]) in the union, in the member's own order, and its faces follow the
+ * previous members' faces.  cell_offsets (n+1 entries) may be NULL.  Physical names are
+ * dropped (face tags are kept).  Stands in for the reference's one-DM-per-run setup when
+ * many runs share one launch. */
+int zdm_mesh_concat(int n, const zdm_mesh* const* meshes, int* cell_offsets, zdm_mesh** out);
+
 void zdm_mesh_destroy(zdm_mesh* m);
 const char* zdm_mesh_last_error(void);
 

@@ -141,3 +141,23 @@ def test_splitmix64_known_values():
     st = ctypes.c_uint64(12345)
     for _ in range(5):
         assert r2.next() == zdm.mesh_lib().zdm_splitmix64_next(ctypes.byref(st))
+
+
+def test_mesh_concat_is_a_disjoint_union():
+    ms = [zdm.Mesh.tri_grid(n, n, jitter=0.1, seed=n) for n in (3, 6, 4)]
+    u, off = zdm.Mesh.concat(ms)
+    assert list(off) == [0, 18, 90, 122] and u.ncells == 122
+    assert u.nfaces == sum(m.nfaces for m in ms)
+    fo = 0
+    for i, m in enumerate(ms):
+        sl = slice(off[i], off[i + 1])
+        assert np.array_equal(u.cell_volume[sl], m.cell_volume)
+        assert np.array_equal(u.cell_nbrs[sl], np.where(m.cell_nbrs >= 0, m.cell_nbrs + off[i], -1))
+        assert np.array_equal(u.cell_faces[sl, :m.max_faces],
+                              np.where(m.cell_faces >= 0, m.cell_faces + fo, -1))
+        fo += m.nfaces
+    # faces stay ordered by (support[0], support[1]) and every neighbour is in its member
+    assert np.all(np.diff(u.face_cells[:, 0]) >= 0)
+    member = np.searchsorted(off, np.arange(u.ncells), side="right") - 1
+    inner = u.face_cells[:, 1] >= 0
+    assert np.array_equal(member[u.face_cells[inner, 0]], member[u.face_cells[inner, 1]])

@@ -68,6 +68,8 @@ def mesh_lib():
                                             ctypes.c_int, _c_int_p, ctypes.c_int, _c_int_p,
                                             _c_int_p, ctypes.c_int, ctypes.c_int, P]
         lib.zdm_mesh_destroy.argtypes = [ctypes.POINTER(_ZdmMesh)]
+        lib.zdm_mesh_concat.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.POINTER(_ZdmMesh)),
+                                        _c_int_p, P]
         lib.zdm_mesh_last_error.restype = ctypes.c_char_p
         lib.zdm_splitmix64_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
         lib.zdm_splitmix64_next.restype = ctypes.c_uint64
@@ -150,6 +152,18 @@ class Mesh:
     def read_msh(cls, path, renumber=True):
         """Gmsh MSH 2.2 (the reference's mesh format, VirtualSolver.hpp:388)."""
         return cls._make(mesh_lib().zdm_mesh_read_msh, path.encode(), _renum(renumber))
+
+    @classmethod
+    def concat(cls, meshes):
+        """Disjoint union (an ensemble); returns (mesh, cell_offsets[n+1])."""
+        n = len(meshes)
+        arr = (ctypes.POINTER(_ZdmMesh) * n)(*[m.handle for m in meshes])
+        off = np.zeros(n + 1, dtype=np.int32)
+        out = ctypes.POINTER(_ZdmMesh)()
+        rc = mesh_lib().zdm_mesh_concat(n, arr, off.ctypes.data_as(_c_int_p), ctypes.byref(out))
+        if rc != 0:
+            raise ZdmError(mesh_lib().zdm_mesh_last_error().decode())
+        return Mesh(out), off
 
     @classmethod
     def from_cells(cls, dim, xyz, cells, bfaces=None, btags=None, mark_all_tag=1, renumber=True):
@@ -254,7 +268,8 @@ SOLVER_SYMBOLS = [
     "zdm_state_upload_device", "zdm_state_download_device", "zdm_enable_timing",
     "zdm_kernel_time", "zdm_stream_copy", "zdm_last_error",
     "zdm_chorin_setup", "zdm_chorin_step", "zdm_chorin_pressure", "zdm_vam_frozen_aux",
-    "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres",
+    "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres", "zdm_batch_create", "zdm_batch_time",
+    "zdm_step_graph",
 ]
 
 
@@ -298,6 +313,11 @@ def model_lib(model):
     lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
     lib.zdm_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _c_dbl_p, _c_dbl_p]
     lib.zdm_last_error.restype = ctypes.c_char_p
+    lib.zdm_batch_create.argtypes = [ctypes.POINTER(_ZdmMesh), _c_int_p, _c_dbl_p,
+                                     ctypes.POINTER(_Options), ctypes.c_int, _c_int_p,
+                                     ctypes.POINTER(V)]
+    lib.zdm_batch_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_int_p]
+    lib.zdm_step_graph.argtypes = [V, ctypes.c_int]
     K = ctypes.POINTER(KrylovOptions)
     lib.zdm_chorin_setup.argtypes = [V, K]
     lib.zdm_chorin_step.argtypes = [V, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
@@ -344,7 +364,7 @@ class Solver:
 
     def __init__(self, model, mesh, settings=None, params=None, face_bc=None, source=True,
                  order=None, ssp_stages=None, cfl=None, t_end=None, gradient=None,
-                 use_limiters=None, well_balanced=None):
+                 use_limiters=None, well_balanced=None, cell_offsets=None):
         self.model = model
         self.info = ModelInfo(model)
         self.lib = model_lib(model)
@@ -385,8 +405,16 @@ class Solver:
                                    else sv.get("well_balanced", False)))
         self.options = o
         h = ctypes.c_void_p()
-        rc = self.lib.zdm_solver_create(mesh.handle, self.face_bc.ctypes.data_as(_c_int_p),
-                                        _dp(self.params), ctypes.byref(o), ctypes.byref(h))
+        if cell_offsets is None:
+            rc = self.lib.zdm_solver_create(mesh.handle, self.face_bc.ctypes.data_as(_c_int_p),
+                                            _dp(self.params), ctypes.byref(o), ctypes.byref(h))
+        else:
+            self.cell_offsets = np.ascontiguousarray(cell_offsets, dtype=np.int32)
+            rc = self.lib.zdm_batch_create(mesh.handle, self.face_bc.ctypes.data_as(_c_int_p),
+                                           _dp(self.params), ctypes.byref(o),
+                                           len(self.cell_offsets) - 1,
+                                           self.cell_offsets.ctypes.data_as(_c_int_p),
+                                           ctypes.byref(h))
         self._check(rc)
         self._h = h
 
@@ -422,6 +450,16 @@ class Solver:
 
     def step(self, n=1):
         self._check(self.lib.zdm_step(self._h, int(n)))
+
+    def step_graph(self, n=1):
+        """n steps by replaying one HIP-graph-captured step."""
+        self._check(self.lib.zdm_step_graph(self._h, int(n)))
+
+    def member_time(self, i):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._check(self.lib.zdm_batch_time(self._h, int(i), ctypes.byref(t), ctypes.byref(dt),
+                                            ctypes.byref(n)))
+        return t.value, dt.value, n.value
 
     def run(self, max_steps=1 << 30, check_every=16):
         n = ctypes.c_int()
@@ -511,6 +549,24 @@ class Solver:
             raise ZdmError(self.lib.zdm_last_error().decode())
         del z
         return rp, col, val
+
+
+class Ensemble(Solver):
+    """Many independent runs of one model in one set of launches (the batching form): the
+    members' meshes are joined into a disjoint union, each member keeps its own dt, time and
+    step count.  Member states are views into the union state."""
+
+    def __init__(self, model, meshes, **kw):
+        union, off = Mesh.concat(meshes)
+        self.members = list(meshes)
+        super().__init__(model, union, cell_offsets=off, **kw)
+
+    def split(self, X):
+        o = self.cell_offsets
+        return [X[o[i]:o[i + 1]] for i in range(len(o) - 1)]
+
+    def join(self, Xs):
+        return np.concatenate(Xs, axis=0)
 
 
 def bsr2_gmres(rowptr, col, val, b, x0=None, krylov=None, model="sme2", **kw):

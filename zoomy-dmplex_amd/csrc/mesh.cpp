@@ -583,6 +583,90 @@ int zdm_mesh_read_msh(const char* path, int renumber, zdm_mesh** out) {
     return 0;
 }
 
+int zdm_mesh_concat(int n, const zdm_mesh* const* ms, int* cell_offsets, zdm_mesh** out) {
+    if (n < 1 || !ms || !out) return fail("zdm_mesh_concat: bad arguments");
+    const int dim = ms[0]->dim;
+    size_t nc = 0, nf = 0, nv = 0;
+    int maxf = 0, maxv = 0;
+    for (int i = 0; i < n; ++i) {
+        const zdm_mesh* a = ms[i];
+        if (!a || a->dim != dim) return fail("zdm_mesh_concat: members must share the dimension");
+        nc += a->ncells;
+        nf += a->nfaces;
+        nv += a->nverts;
+        maxf = std::max(maxf, a->max_faces);
+        maxv = std::max(maxv, a->max_cell_verts);
+    }
+    if (nc > (size_t)INT32_MAX / 4 || nf > (size_t)INT32_MAX / 2 || nv > (size_t)INT32_MAX / 3)
+        return fail("zdm_mesh_concat: union too large for 32-bit indices");
+    std::vector<double> ccen(nc * 3), cvol(nc), fnorm(nf * 3), fcen(nf * 3), verts(nv * 3);
+    std::vector<int> cnf(nc), cfaces(nc * maxf, -1), cnbrs(nc * maxf, -1), cperm(nc),
+        cverts(nc * maxv, -1), fcells(nf * 2), ftag(nf);
+    size_t oc = 0, of = 0, ov = 0;
+    for (int i = 0; i < n; ++i) {
+        const zdm_mesh* a = ms[i];
+        if (cell_offsets) cell_offsets[i] = (int)oc;
+        const int MF = a->max_faces, MV = a->max_cell_verts;
+        for (int c = 0; c < a->ncells; ++c) {
+            const size_t g = oc + c;
+            for (int d = 0; d < 3; ++d) ccen[3 * g + d] = a->cell_centroid[3 * (size_t)c + d];
+            cvol[g] = a->cell_volume[c];
+            cnf[g] = a->cell_nfaces[c];
+            cperm[g] = (int)oc + a->cell_perm[c];
+            for (int k = 0; k < MF; ++k) {
+                const int f = a->cell_faces[(size_t)c * MF + k], nb = a->cell_nbrs[(size_t)c * MF + k];
+                cfaces[g * maxf + k] = f < 0 ? -1 : (int)of + f;
+                cnbrs[g * maxf + k] = nb < 0 ? -1 : (int)oc + nb;
+            }
+            for (int k = 0; k < MV; ++k) {
+                const int v = a->cell_verts[(size_t)c * MV + k];
+                cverts[g * maxv + k] = v < 0 ? -1 : (int)ov + v;
+            }
+        }
+        for (int f = 0; f < a->nfaces; ++f) {
+            const size_t g = of + f;
+            const int L = a->face_cells[2 * f], R = a->face_cells[2 * f + 1];
+            fcells[2 * g] = (int)oc + L;
+            fcells[2 * g + 1] = R < 0 ? -1 : (int)oc + R;
+            for (int d = 0; d < 3; ++d) {
+                fnorm[3 * g + d] = a->face_normal[3 * (size_t)f + d];
+                fcen[3 * g + d] = a->face_centroid[3 * (size_t)f + d];
+            }
+            ftag[g] = a->face_tag[f];
+        }
+        for (size_t v = 0; v < 3 * (size_t)a->nverts; ++v) verts[3 * ov + v] = a->verts[v];
+        oc += a->ncells;
+        of += a->nfaces;
+        ov += a->nverts;
+    }
+    if (cell_offsets) cell_offsets[n] = (int)oc;
+    zdm_mesh* m = static_cast<zdm_mesh*>(std::calloc(1, sizeof(zdm_mesh)));
+    m->dim = dim;
+    m->ncells = (int)nc;
+    m->nfaces = (int)nf;
+    m->max_faces = maxf;
+    m->nverts = (int)nv;
+    m->max_cell_verts = maxv;
+    m->cell_centroid = dup(ccen);
+    m->cell_volume = dup(cvol);
+    m->cell_nfaces = dup(cnf);
+    m->cell_faces = dup(cfaces);
+    m->cell_nbrs = dup(cnbrs);
+    m->cell_perm = dup(cperm);
+    m->cell_verts = dup(cverts);
+    m->face_cells = dup(fcells);
+    m->face_normal = dup(fnorm);
+    m->face_centroid = dup(fcen);
+    m->face_tag = dup(ftag);
+    m->verts = dup(verts);
+    m->nphysical = 0;
+    m->physical_tag = static_cast<int*>(std::malloc(sizeof(int)));
+    m->physical_dim = static_cast<int*>(std::malloc(sizeof(int)));
+    m->physical_names = static_cast<char*>(std::calloc(1, 64));
+    *out = m;
+    return 0;
+}
+
 void zdm_mesh_destroy(zdm_mesh* m) {
     if (!m) return;
     std::free(m->cell_centroid);

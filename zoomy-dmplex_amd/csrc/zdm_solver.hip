@@ -114,16 +114,41 @@ struct DevMesh {
     const T* fnhy;
     const int* fbc;   // model BC index of boundary faces, -1 = none
     const T* lsqw;    // [MAXF][DIM][ld] least-squares weights
+    // ensemble of nmem member meshes in one launch (nmem = 0: a single mesh): each member owns
+    // whole workgroups, blocks [blk_first[m], blk_first[m+1]) cover its cells
+    // [cell_off[m], cell_off[m+1])
+    const int* blk_member;
+    const int* blk_first;
+    const int* cell_off;
+    int nmem;
 };
 
-__device__ __forceinline__ int cell_index() {
+__device__ __forceinline__ int logical_block() {
     // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give each XCD a
     // contiguous range of workgroups so face neighbours share that XCD's L2.
     const int nb = gridDim.x;
     const int xcd = blockIdx.x % NXCD, idx = blockIdx.x / NXCD;
     const int q = nb / NXCD, r = nb % NXCD;
-    const int lb = xcd * q + (xcd < r ? xcd : r) + idx;
-    return lb * blockDim.x + threadIdx.x;
+    return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+__device__ __forceinline__ int cell_index() { return logical_block() * blockDim.x + threadIdx.x; }
+
+// cell, logical block and ensemble member of this thread; c = ncells past a member's end
+struct Loc {
+    int c, lb, m;
+};
+__device__ __forceinline__ Loc locate(const DevMesh& m) {
+    Loc L;
+    L.lb = logical_block();
+    if (m.nmem == 0) {
+        L.m = 0;
+        L.c = L.lb * blockDim.x + threadIdx.x;
+        return L;
+    }
+    L.m = m.blk_member[L.lb];
+    const int c = m.cell_off[L.m] + (L.lb - m.blk_first[L.m]) * blockDim.x + threadIdx.x;
+    L.c = c < m.cell_off[L.m + 1] ? c : m.ncells;
+    return L;
 }
 
 template <int N>
@@ -156,8 +181,10 @@ template <int MAXF, int GRAD>
 __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 const T* __restrict__ X, const T* __restrict__ Ag,
                                                 T* __restrict__ As, T* __restrict__ QF, int use_limiters) {
+    const Loc L = locate(m);
+    ctrl += L.m;
     if (!ctrl->active) return;
-    const int c = cell_index();
+    const int c = L.c;
     if (c >= m.ncells) return;
     const int ld = m.ld;
     T q[NQ], qc[NQ], ac[NA];
@@ -322,8 +349,10 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
                                                 int stage, const T* __restrict__ QF,
                                                 const T* __restrict__ As, const T* In, T* Out,
                                                 const T* __restrict__ Ag, int wb) {
+    const Loc L = locate(m);
+    ctrl += L.m;
     if (!ctrl->active) return;
-    const int c = cell_index();
+    const int c = L.c;
     if (c >= m.ncells) return;
     const int ld = m.ld;
     const T time = ctrl->stage_t[stage];
@@ -576,8 +605,10 @@ __device__ __forceinline__ T block_min(T v) {
 template <int POST>
 __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl* __restrict__ ctrl, T* X,
                                                 T* A, T* __restrict__ partials, T dt_override) {
+    const Loc L = locate(m);
+    ctrl += L.m;
     if (!ctrl->active) return;
-    const int c = cell_index();
+    const int c = L.c;
     const int ld = m.ld;
     T ratio = DBL_MAX;
     if (c < m.ncells) {
@@ -598,18 +629,24 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
     }
     if constexpr (POST <= 2) {
         ratio = block_min(ratio);
-        if (threadIdx.x == 0) partials[blockIdx.x] = ratio;
+        if (threadIdx.x == 0) partials[L.lb] = ratio;
     }
 }
 
 // what: 0 = standalone ComputeTimeStep (-> dt_query), 1 = first dt, 2 = end of a step,
 //       3 = end of a Chorin step (next dt = max(CFL, min_dt), ChorinVAM.hpp:103)
+// One workgroup per ensemble member (one in all for a single mesh): the member's partial minima
+// are those of its own workgroups.
 __global__ __launch_bounds__(BLOCK) void k_finalize(Ctrl* ctrl, const T* __restrict__ partials, int np,
-                                                    int what, int order, int s, T cfl, T min_dt,
-                                                    T t_end, T dt_max) {
-    if (what == 2 && !ctrl->active) return;
+                                                    const int* __restrict__ blk_first, int what,
+                                                    int order, int s, T cfl, T min_dt, T t_end,
+                                                    T dt_max) {
+    ctrl += blockIdx.x;
+    if ((what == 2 || what == 3) && !ctrl->active) return;
+    const int b0 = blk_first ? blk_first[blockIdx.x] : 0;
+    const int b1 = blk_first ? blk_first[blockIdx.x + 1] : np;
     T v = DBL_MAX;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) v = partials[i] < v ? partials[i] : v;
+    for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) v = partials[i] < v ? partials[i] : v;
     v = block_min(v);
     if (threadIdx.x != 0) return;
     const bool fin = !(!isfinite(v) || v >= 0.5 * DBL_MAX);
@@ -702,6 +739,10 @@ struct zdm_solver {
     double kt_ms[3];
     int kt_n[3];
     ChorinHost* ch;                   // pressure split state (VAM models)
+    int nmem;                         // ensemble members (0: one mesh)
+    int nctrl;                        // control blocks: max(1, nmem)
+    hipGraph_t graph;                 // one captured time step (zdm_step_graph)
+    hipGraphExec_t graph_exec;
 };
 
 namespace {
@@ -800,9 +841,9 @@ void post(zdm_solver* s, int mode, T dt_override = 0) {
 }
 
 void finalize(zdm_solver* s, int what) {
-    k_finalize<<<1, BLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, what, s->opt.order,
-                                          s->opt.ssp_stages, s->opt.cfl, s->opt.min_dt, s->opt.t_end,
-                                          (T)M::dt_max);
+    k_finalize<<<s->nctrl, BLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, s->dm.nmem ? s->dm.blk_first : nullptr,
+                                                 what, s->opt.order, s->opt.ssp_stages, s->opt.cfl,
+                                                 s->opt.min_dt, s->opt.t_end, (T)M::dt_max);
 }
 
 #ifdef ZOOMY_MODEL_IS_VAM
@@ -813,12 +854,23 @@ void chorin_free(ChorinHost*) {}
 #endif
 
 int set_ctrl_rhs_time(zdm_solver* s, T t) {
-    Ctrl c;
-    std::memset(&c, 0, sizeof(c));
-    c.active = 1;
-    c.t = t;
-    c.stage_t[0] = t;
-    HIPCHK(hipMemcpyAsync(s->ctrl_rhs, &c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    std::vector<Ctrl> cs(s->nctrl);
+    for (Ctrl& c : cs) {
+        std::memset(&c, 0, sizeof(c));
+        c.active = 1;
+        c.t = t;
+        c.stage_t[0] = t;
+    }
+    HIPCHK(hipMemcpyAsync(s->ctrl_rhs, cs.data(), sizeof(Ctrl) * cs.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// all control blocks (one per member) to the host
+int read_ctrls(zdm_solver* s, std::vector<Ctrl>& cs) {
+    cs.resize(s->nctrl);
+    HIPCHK(hipMemcpyAsync(cs.data(), s->ctrl, sizeof(Ctrl) * cs.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
 }
 
@@ -861,8 +913,8 @@ int zdm_model_parameter_names(char* names64, int cap) {
     return (int)v.size();
 }
 
-int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* params,
-                      const zdm_options* opts, zdm_solver** out) {
+static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* params,
+                       const zdm_options* opts, int nmem, const int* cell_off, zdm_solver** out) {
     if (!mesh || !face_bc || !opts || !out) return fail("zdm_solver_create: null argument");
     if (mesh->dim != DIM) return fail("mesh dimension does not match Model::dimension");
     if (mesh->max_faces > 4 || mesh->max_faces < 1) return fail("unsupported faces per cell");
@@ -885,6 +937,19 @@ int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* pa
     s->maxf = MF;
     s->ld = (int)align_up((size_t)nc, 256);
     s->nblocks = (nc + BLOCK - 1) / BLOCK;
+    s->nmem = nmem;
+    s->nctrl = nmem > 0 ? nmem : 1;
+    std::vector<int> blk_member, blk_first, coff;
+    if (nmem > 0) {
+        blk_first.assign(nmem + 1, 0);
+        coff.assign(cell_off, cell_off + nmem + 1);
+        for (int i = 0; i < nmem; ++i) {
+            const int cnt = coff[i + 1] - coff[i];
+            blk_first[i + 1] = blk_first[i] + (cnt + BLOCK - 1) / BLOCK;
+            for (int b = blk_first[i]; b < blk_first[i + 1]; ++b) blk_member.push_back(i);
+        }
+        s->nblocks = blk_first[nmem];
+    }
     const int ld = s->ld;
     const int MFa = MF <= 2 ? 2 : MF;
 
@@ -980,7 +1045,10 @@ int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* pa
     size_t oQF = take(sizeof(T) * (size_t)MFa * NQ * ld);
     size_t oPart = take(sizeof(T) * (size_t)s->nblocks);
     size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
-    size_t oCtrl = take(sizeof(Ctrl)), oCtrlR = take(sizeof(Ctrl));
+    size_t oCtrl = take(sizeof(Ctrl) * s->nctrl), oCtrlR = take(sizeof(Ctrl) * s->nctrl);
+    size_t oBm = take(sizeof(int) * std::max<size_t>(blk_member.size(), 1)),
+           oBf = take(sizeof(int) * std::max<size_t>(blk_first.size(), 1)),
+           oCo = take(sizeof(int) * std::max<size_t>(coff.size(), 1));
     size_t oVol = take(sizeof(T) * ld), oCx = take(sizeof(T) * ld), oCy = take(sizeof(T) * ld), oRc = take(sizeof(T) * ld);
     size_t oCface = take(sizeof(int) * (size_t)MFa * ld), oCnbr = take(sizeof(int) * (size_t)MFa * ld),
            oCmeta = take(sizeof(int) * (size_t)MFa * ld);
@@ -1026,6 +1094,11 @@ int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* pa
     for (int i = 0; i < 7 && nf > 0; ++i) e = e != hipSuccess ? e : up(oF7[i], fa[i]->data(), sizeof(T) * nf);
     if (nf > 0) e = e != hipSuccess ? e : up(oFbc, fbc.data(), sizeof(int) * nf);
     if (!lsqw.empty()) e = e != hipSuccess ? e : up(oLsq, lsqw.data(), sizeof(T) * lsqw.size());
+    if (nmem > 0) {
+        e = e != hipSuccess ? e : up(oBm, blk_member.data(), sizeof(int) * blk_member.size());
+        e = e != hipSuccess ? e : up(oBf, blk_first.data(), sizeof(int) * blk_first.size());
+        e = e != hipSuccess ? e : up(oCo, coff.data(), sizeof(int) * coff.size());
+    }
     if (e != hipSuccess) {
         hipFree(s->arena);
         hipStreamDestroy(s->stream);
@@ -1052,17 +1125,21 @@ int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* pa
     d.fnhy = (T*)(a + oF7[6]);
     d.fbc = (int*)(a + oFbc);
     d.lsqw = (T*)(a + oLsq);
+    d.blk_member = (int*)(a + oBm);
+    d.blk_first = (int*)(a + oBf);
+    d.cell_off = (int*)(a + oCo);
+    d.nmem = nmem;
     s->timing = false;
     s->evused = 0;
     for (int i = 0; i < 3; ++i) {
         s->kt_ms[i] = 0;
         s->kt_n[i] = 0;
     }
-    Ctrl c0;
-    std::memset(&c0, 0, sizeof(c0));
-    hipMemcpy(s->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice);
+    std::vector<Ctrl> c0(s->nctrl);
+    std::memset(c0.data(), 0, sizeof(Ctrl) * c0.size());
+    hipMemcpy(s->ctrl, c0.data(), sizeof(Ctrl) * c0.size(), hipMemcpyHostToDevice);
 #ifdef ZOOMY_MODEL_IS_VAM
-    if (chorin_build(s, mesh, krylov_defaults())) {
+    if (nmem == 0 && chorin_build(s, mesh, krylov_defaults())) {
         const std::string msg = g_err;
         zdm_solver_destroy(s);
         return fail(msg);
@@ -1072,9 +1149,35 @@ int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* pa
     return 0;
 }
 
+int zdm_solver_create(const zdm_mesh* mesh, const int* face_bc, const double* params,
+                      const zdm_options* opts, zdm_solver** out) {
+    return create_impl(mesh, face_bc, params, opts, 0, nullptr, out);
+}
+
+int zdm_batch_create(const zdm_mesh* mesh, const int* face_bc, const double* params,
+                     const zdm_options* opts, int nmembers, const int* cell_offsets, zdm_solver** out) {
+    if (!mesh || nmembers < 1 || !cell_offsets) return fail("zdm_batch_create: bad arguments");
+    if (cell_offsets[0] != 0 || cell_offsets[nmembers] != mesh->ncells)
+        return fail("zdm_batch_create: cell offsets must run from 0 to ncells");
+    for (int i = 0; i < nmembers; ++i)
+        if (cell_offsets[i + 1] <= cell_offsets[i]) return fail("zdm_batch_create: every member needs cells");
+    // members must not share faces: every neighbour of a member cell is in the member
+    const int MF = mesh->max_faces;
+    for (int i = 0; i < nmembers; ++i)
+        for (int c = cell_offsets[i]; c < cell_offsets[i + 1]; ++c)
+            for (int k = 0; k < mesh->cell_nfaces[c]; ++k) {
+                const int nb = mesh->cell_nbrs[(size_t)c * MF + k];
+                if (nb >= 0 && (nb < cell_offsets[i] || nb >= cell_offsets[i + 1]))
+                    return fail("zdm_batch_create: members are coupled by a face");
+            }
+    return create_impl(mesh, face_bc, params, opts, nmembers, cell_offsets, out);
+}
+
 int zdm_solver_destroy(zdm_solver* s) {
     if (!s) return 0;
     hipStreamSynchronize(s->stream);
+    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) hipGraphDestroy(s->graph);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
     chorin_free(s->ch);
     hipFree(s->arena);
@@ -1146,10 +1249,10 @@ int zdm_state_download_device(zdm_solver* s, double* dX, double* dA) {
 
 int zdm_initialize(zdm_solver* s) {
     if (!s) return fail("null solver");
-    Ctrl c0;
-    std::memset(&c0, 0, sizeof(c0));
-    c0.active = 1;
-    HIPCHK(hipMemcpyAsync(s->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    std::vector<Ctrl> c0(s->nctrl);
+    std::memset(c0.data(), 0, sizeof(Ctrl) * c0.size());
+    for (Ctrl& c : c0) c.active = 1;
+    HIPCHK(hipMemcpyAsync(s->ctrl, c0.data(), sizeof(Ctrl) * c0.size(), hipMemcpyHostToDevice, s->stream));
     post(s, 1);
     finalize(s, 1);
     HIPCHK(hipGetLastError());
@@ -1186,36 +1289,59 @@ static int resolve_events(zdm_solver* s) {
     return 0;
 }
 
+// the launches of one MUSCLSolver time step
+static void enqueue_step(zdm_solver* s) {
+    const int order = s->opt.order, st = s->opt.ssp_stages;
+    record(s, -1);
+    if (order < 2) {
+        prep(s, s->ctrl, s->X, s->A);
+        record(s, 0);
+        flux(s, s->ctrl, MODE_EULER, 0, s->X, s->X, s->A);
+        record(s, 1);
+    } else {
+        for (int i = 0; i < st - 1; ++i) {
+            const T* in = i == 0 ? s->X : s->W;
+            prep(s, s->ctrl, in, s->A);
+            record(s, 0);
+            flux(s, s->ctrl, MODE_STAGE, i, in, s->W, s->A);
+            record(s, 1);
+        }
+        prep(s, s->ctrl, s->W, s->A);
+        record(s, 0);
+        flux(s, s->ctrl, MODE_FINAL, st - 1, s->W, s->X, s->A);
+        record(s, 1);
+    }
+    post(s, s->opt.source ? 2 : 1);
+    record(s, 2);
+    finalize(s, 2);
+}
+
 int zdm_step(zdm_solver* s, int n) {
     if (!s) return fail("null solver");
-    const int order = s->opt.order, st = s->opt.ssp_stages;
     for (int step = 0; step < n; ++step) {
         if (s->timing && s->evpool.size() > 4096 && s->evused + 64 > s->evpool.size())
             if (resolve_events(s)) return -1;
-        record(s, -1);
-        if (order < 2) {
-            prep(s, s->ctrl, s->X, s->A);
-            record(s, 0);
-            flux(s, s->ctrl, MODE_EULER, 0, s->X, s->X, s->A);
-            record(s, 1);
-        } else {
-            for (int i = 0; i < st - 1; ++i) {
-                const T* in = i == 0 ? s->X : s->W;
-                prep(s, s->ctrl, in, s->A);
-                record(s, 0);
-                flux(s, s->ctrl, MODE_STAGE, i, in, s->W, s->A);
-                record(s, 1);
-            }
-            prep(s, s->ctrl, s->W, s->A);
-            record(s, 0);
-            flux(s, s->ctrl, MODE_FINAL, st - 1, s->W, s->X, s->A);
-            record(s, 1);
-        }
-        post(s, s->opt.source ? 2 : 1);
-        record(s, 2);
-        finalize(s, 2);
+        enqueue_step(s);
         HIPCHK(hipGetLastError());
     }
+    return 0;
+}
+
+int zdm_step_graph(zdm_solver* s, int n) {
+    if (!s) return fail("null solver");
+    if (!s->graph_exec) {
+        const bool was = s->timing;
+        s->timing = false;
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->stream, &g);
+        s->timing = was;
+        if (e != hipSuccess) return fail(std::string("step capture failed: ") + hipGetErrorString(e));
+        s->graph = g;
+        HIPCHK(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(s->graph_exec, s->stream));
     return 0;
 }
 
@@ -1258,14 +1384,25 @@ int zdm_run(zdm_solver* s, int max_steps, int check_every, int* steps_done) {
         int n = std::min(check_every, max_steps - done);
         if (zdm_step(s, n)) return -1;
         done += n;
-        Ctrl c;
-        HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
-        HIPCHK(hipStreamSynchronize(s->stream));
-        if (!c.active) break;
+        std::vector<Ctrl> cs;
+        if (read_ctrls(s, cs)) return -1;
+        bool any = false;
+        for (const Ctrl& c : cs) any = any || c.active;
+        if (!any) break;
     }
     int st1 = 0;
     if (zdm_get_time(s, nullptr, nullptr, &st1)) return -1;
     if (steps_done) *steps_done = st1 - st0;
+    return 0;
+}
+
+int zdm_batch_time(zdm_solver* s, int member, double* time, double* dt, int* steps) {
+    if (!s || member < 0 || member >= s->nctrl) return fail("zdm_batch_time: bad member");
+    std::vector<Ctrl> cs;
+    if (read_ctrls(s, cs)) return -1;
+    if (time) *time = cs[member].t;
+    if (dt) *dt = cs[member].dt_cfl;
+    if (steps) *steps = cs[member].steps;
     return 0;
 }
 
@@ -1344,6 +1481,7 @@ int zdm_stream_copy(size_t bytes, int iters, double* gbs, double* avg_ms) {
 
 int zdm_cfl_dt(zdm_solver* s, double* dt) {
     if (!s || !dt) return fail("null argument");
+    if (s->nmem > 0) return fail("zdm_cfl_dt: use zdm_batch_time for an ensemble");
     // the standalone query must not disturb an inactive run: force the CFL pass on
     Ctrl c;
     HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
@@ -1468,7 +1606,8 @@ int zdm_chorin_setup(zdm_solver* s, const zdm_krylov_options* opts) {
 }
 
 int zdm_chorin_step(zdm_solver* s, int n, int* steps_done, int* its_total, int* last_reason) {
-    if (!s || !s->ch) return fail("null solver");
+    if (!s) return fail("null solver");
+    if (!s->ch) return fail("zdm_chorin_step: the pressure split runs on a single mesh, not an ensemble");
     int done = 0, its = 0, reason = 0;
     for (int i = 0; i < n; ++i) {
         const int r = chorin_one_step(s);
