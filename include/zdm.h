@@ -170,6 +170,31 @@ int zdm_vam_pmat(zdm_solver* s, double dt, int cap, int* rowptr, int* col, doubl
 int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val, const double* b,
                    double* x, const zdm_krylov_options* opts, int* its, int* reason);
 
+/* ---- Discrete adjoint of the time step (the differentiation form).  A calibration problem
+ * of nsteps MUSCLSolver steps from (X0, A0) with the loss
+ *     L = sum_{k>=1} sum_g (h(gauges[g]) after step k*obs_every - obs[(k-1)*ngauge + g])^2,
+ * and its gradient with respect to the initial state X0 (bathymetry = slot 0) and the
+ * initial aux A0 (pass-through aux such as a per-cell friction coefficient).  The gradient
+ * is the exact transpose of the discrete step as computed, with the branches of clamps,
+ * bounds, limiter and wet/dry switches and the time steps frozen at the forward run.
+ * Memory: a checkpoint every checkpoint_every steps (<= 0: ceil(sqrt(nsteps))) plus one
+ * segment of states; the reverse sweep re-runs each segment once.  Order 1, or order 2 with
+ * the Green-Gauss gradient; not for ensembles or the well-balanced extension.  No
+ * counterpart in the reference, which has no differentiation path. */
+int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
+                      const int* gauges, const double* obs);
+/* Forward run (X0, A0 in the zdm_state_upload layout, A0 may be NULL).  dts = NULL takes
+ * dt = CFL each step and records it; otherwise the given sequence is used (a frozen-dt
+ * re-run, e.g. for finite differences).  Returns the loss. */
+int zdm_adjoint_forward(zdm_solver* s, const double* X0, const double* A0, const double* dts,
+                        double* loss);
+/* Gradient of the loss of the last forward run: gX0 (ncells*n_q), gA0 (ncells*n_aux);
+ * either may be NULL. */
+int zdm_adjoint_gradient(zdm_solver* s, double* gX0, double* gA0);
+/* The dt sequence (nsteps) and the residuals h - obs (nobs*ngauge) of the last forward run. */
+int zdm_adjoint_dts(zdm_solver* s, double* dts);
+int zdm_adjoint_residuals(zdm_solver* s, double* r);
+
 const char* zdm_last_error(void);
 
 #ifdef __cplusplus

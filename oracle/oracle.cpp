@@ -25,6 +25,9 @@
 
 #include "Numerics.H"
 #include "zdm_mesh.h"
+#ifdef ORACLE_TANGENT
+#include "dual.H"
+#endif
 
 template <typename TT, typename = void>
 struct HasCellSource : std::false_type {};
@@ -36,12 +39,26 @@ struct HasCellSource<TT, std::void_t<decltype(Model<TT>::hydrostatic_cell_source
 
 namespace {
 
+#ifdef ORACLE_TANGENT
+// tangent-linear build: every quantity carries one directional derivative
+using T = Dual<1>;
+#else
 using T = double;
+#endif
 using M = Model<T>;
 using NUM = Numerics<T>;
 constexpr int NQ = M::n_dof_q;
 constexpr int NA = M::n_dof_qaux;
 constexpr int DIM = M::dimension;
+
+// mesh coordinates (always double) as scalars of the build
+struct P3 {
+    T v[3];
+    explicit P3(const double* p) {
+        for (int d = 0; d < 3; ++d) v[d] = p[d];
+    }
+};
+const T BIG = (T)std::numeric_limits<double>::max();
 
 struct Opts {
     int order;            // 1 = PCM + forward Euler, 2 = MUSCL + SSP-RK
@@ -96,7 +113,7 @@ void neighbor_bounds(const zdm_mesh* m, const T* X, T* vmin, T* vmax) {
 void gradient_gg(const zdm_mesh* m, const int* face_bc, const T* p, const T* X, T* G) {
     std::memset(G, 0, sizeof(T) * (size_t)m->ncells * NQ * DIM);
     for (int f = 0; f < m->nfaces; ++f) {
-        const T* N = m->face_normal + 3 * (size_t)f;
+        const double* N = m->face_normal + 3 * (size_t)f;
         const int cL = m->face_cells[2 * f], cR = m->face_cells[2 * f + 1];
         if (cR >= 0) {
             const T* qL = X + (size_t)cL * NQ;
@@ -119,7 +136,7 @@ void gradient_gg(const zdm_mesh* m, const int* face_bc, const T* p, const T* X, 
             T nh[3] = {0, 0, 0};
             for (int d = 0; d < DIM; ++d) nh[d] = N[d] / area;
             auto qb = M::boundary_conditions(face_bc[f], qL, nullptr, p, nh,
-                                             m->face_centroid + 3 * (size_t)f, 0.0, 0.0);
+                                             P3(m->face_centroid + 3 * (size_t)f).v, 0.0, 0.0);
             T* gL = G + (size_t)cL * NQ * DIM;
             for (int i = 0; i < NQ; ++i)
                 for (int d = 0; d < DIM; ++d) gL[i * DIM + d] += qb[i] * N[d];
@@ -164,13 +181,13 @@ bool chol_solve(int n, std::vector<T>& A, std::vector<T>& b) {
 void gradient_lsq(const zdm_mesh* m, const T* X, T* G) {
     std::memset(G, 0, sizeof(T) * (size_t)m->ncells * NQ * DIM);
     for (int c = 0; c < m->ncells; ++c) {
-        const T* xc = m->cell_centroid + 3 * (size_t)c;
+        const double* xc = m->cell_centroid + 3 * (size_t)c;
         std::vector<int> nb;
         std::vector<std::vector<T>> bv;
         for (int k = 0; k < m->cell_nfaces[c]; ++k) {
             int n = m->cell_nbrs[(size_t)c * m->max_faces + k];
             if (n < 0) continue;
-            const T* xn = m->cell_centroid + 3 * (size_t)n;
+            const double* xn = m->cell_centroid + 3 * (size_t)n;
             std::vector<T> b(DIM);
             for (int d = 0; d < DIM; ++d) b[d] = xn[d] - xc[d];
             nb.push_back(n);
@@ -196,8 +213,8 @@ void gradient_lsq(const zdm_mesh* m, const T* X, T* G) {
 
 // LinearReconstructor::Reconstruct with VacuumLimiter + TVDLimiter,
 // Reconstruction.hpp:29-35, 64-80, 203-243; PCMReconstructor 147-149 when grad == nullptr
-void reconstruct(const T* q, const T* grad, const T* xc, const T* xf, const T* qmin, const T* qmax,
-                 int use_limiters, T* out) {
+void reconstruct(const T* q, const T* grad, const double* xc, const double* xf, const T* qmin,
+                 const T* qmax, int use_limiters, T* out) {
     if (!grad) {
         for (int i = 0; i < NQ; ++i) out[i] = q[i];
         return;
@@ -266,8 +283,9 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
                     const T* A, const T* G, const T* vmin, const T* vmax, int use_limiters, int wb,
                     T* F) {
     for (int f = 0; f < m->nfaces; ++f) {
-        const T* N = m->face_normal + 3 * (size_t)f;
-        const T* xf = m->face_centroid + 3 * (size_t)f;
+        const double* N = m->face_normal + 3 * (size_t)f;
+        const double* xf = m->face_centroid + 3 * (size_t)f;
+        const P3 xfT(xf);
         T area = 0;
         for (int d = 0; d < DIM; ++d) area += N[d] * N[d];
         area = std::sqrt(area);
@@ -287,9 +305,9 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
                 aLf[i] = A[(size_t)cL * NA + i];
                 aRf[i] = A[(size_t)cR * NA + i];
             }
-            auto raL = M::update_aux_variables(qLf, aLf, p, 0.0, xf);
+            auto raL = M::update_aux_variables(qLf, aLf, p, 0.0, xfT.v);
             for (int i = 0; i < NA; ++i) aLf[i] = raL[i];
-            auto raR = M::update_aux_variables(qRf, aRf, p, 0.0, xf);
+            auto raR = M::update_aux_variables(qRf, aRf, p, 0.0, xfT.v);
             for (int i = 0; i < NA; ++i) aRf[i] = raR[i];
             auto flux = NUM::numerical_flux(qLf, qRf, aLf, aRf, p, nh);
             T* fL = F + (size_t)cL * NQ;
@@ -311,10 +329,10 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
                         m->cell_centroid + 3 * (size_t)cL, xf, vmin ? vmin + (size_t)cL * NQ : nullptr,
                         vmax ? vmax + (size_t)cL * NQ : nullptr, use_limiters, qLf);
             for (int i = 0; i < NA; ++i) aLf[i] = A[(size_t)cL * NA + i];
-            auto raL = M::update_aux_variables(qLf, aLf, p, time, xf);
+            auto raL = M::update_aux_variables(qLf, aLf, p, time, xfT.v);
             for (int i = 0; i < NA; ++i) aLf[i] = raL[i];
-            auto qR = M::boundary_conditions(face_bc[f], qLf, aLf, p, nh, xf, time, 0.0);
-            auto raR = M::update_aux_variables(qR.data, aLf, p, time, xf);
+            auto qR = M::boundary_conditions(face_bc[f], qLf, aLf, p, nh, xfT.v, time, 0.0);
+            auto raR = M::update_aux_variables(qR.data, aLf, p, time, xfT.v);
             for (int i = 0; i < NA; ++i) aRf[i] = raR[i];
             auto flux = NUM::numerical_flux(qLf, qR.data, aLf, aRf, p, nh);
             T* fL = F + (size_t)cL * NQ;
@@ -368,7 +386,7 @@ void form_rhs(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, 
 
 // VirtualSolver::ComputeTimeStep, VirtualSolver.hpp:285-357
 T compute_time_step(const zdm_mesh* m, const T* p, const T* X, const T* A, T cfl, int* finite) {
-    T local_min = std::numeric_limits<T>::max();
+    T local_min = BIG;
     for (int c = 0; c < m->ncells; ++c) {
         const T* qc = X + (size_t)c * NQ;
         const T* ac = A + (size_t)c * NA;
@@ -380,10 +398,10 @@ T compute_time_step(const zdm_mesh* m, const T* p, const T* X, const T* A, T cfl
             if (r[0] > eig) eig = r[0];
         }
         if (eig < 1e-14) continue;
-        const T* xc = m->cell_centroid + 3 * (size_t)c;
-        T rc = std::numeric_limits<T>::max();
+        const double* xc = m->cell_centroid + 3 * (size_t)c;
+        T rc = BIG;
         for (int k = 0; k < m->cell_nfaces[c]; ++k) {
-            const T* xf = m->face_centroid + 3 * (size_t)m->cell_faces[(size_t)c * m->max_faces + k];
+            const double* xf = m->face_centroid + 3 * (size_t)m->cell_faces[(size_t)c * m->max_faces + k];
             T d2 = 0.0;
             for (int d = 0; d < DIM; ++d) {
                 T dd = xc[d] - xf[d];
@@ -392,12 +410,12 @@ T compute_time_step(const zdm_mesh* m, const T* p, const T* X, const T* A, T cfl
             T dist = std::sqrt(d2);
             if (dist < rc) rc = dist;
         }
-        if (rc < std::numeric_limits<T>::max()) {
+        if (rc < BIG) {
             T ratio = rc / eig;
             if (ratio < local_min) local_min = ratio;
         }
     }
-    if (!std::isfinite(local_min) || local_min >= 0.5 * std::numeric_limits<T>::max()) {
+    if (!std::isfinite(local_min) || local_min >= 0.5 * BIG) {
         if (finite) *finite = 0;
         return M::dt_max;
     }
@@ -530,13 +548,52 @@ void ts_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T
 
 }  // namespace
 
+namespace {
+
+// Calibration loss with a frozen dt sequence (MUSCLSolver loop, MUSCLSolver.hpp:140-188):
+// constraints once, then per step TSStep, constraints, source, constraints;
+//   loss = sum over steps k*obs_every (k >= 1) and gauges g of (h_g - obs[k-1][g])^2.
+// Run on double (orc_loss) and on one-direction dual numbers (orc_tl_loss, the
+// tangent-linear reference for the discrete adjoint).
+T run_loss(const zdm_mesh* m, const int* face_bc, const T* P, int order, int ssp_stages,
+           int use_limiters, int nsteps, const double* dts, T* X, T* A, int ngauge,
+           const int* gauges, int obs_every, const double* obs) {
+    const int nc = m->ncells;
+    Opts o{order, ssp_stages, use_limiters, 0, 0, 0.5, 1e300, 0.0};
+    Work w;
+    enforce(nc, P, X, A);
+    T L = 0;
+    double t = 0;
+    for (int n = 0; n < nsteps; ++n) {
+        const T dt = T(dts[n]);
+        ts_step(m, face_bc, P, o, T(t), dt, X, A, w);
+        enforce(nc, P, X, A);
+        source_solve(nc, P, dt, X, A);
+        enforce(nc, P, X, A);
+        t += dts[n];
+        if (obs_every > 0 && (n + 1) % obs_every == 0) {
+            const int k = (n + 1) / obs_every - 1;
+            for (int g = 0; g < ngauge; ++g) {
+                const T r = X[(size_t)gauges[g] * NQ + 1] - T(obs[(size_t)k * ngauge + g]);
+                L += r * r;
+            }
+        }
+    }
+    return L;
+}
+
+}  // namespace
+
 #ifdef ZOOMY_MODEL_IS_VAM
 #include "ChorinOps.H"
 namespace {
+#ifndef ORACLE_TANGENT
 #include "oracle_chorin.inc"
+#endif
 }  // namespace
 #endif
 
+#ifndef ORACLE_TANGENT
 extern "C" {
 
 int orc_model_info(int* nq, int* naux, int* npar, int* dim) {
@@ -667,3 +724,40 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
 }
 
 }  // extern "C"
+
+#endif  // ORACLE_TANGENT
+
+#ifdef ORACLE_TANGENT
+// Tangent-linear run: the loss and its directional derivative along (dX0, dA0).
+extern "C" int orc_tl_loss(const zdm_mesh* m, const int* face_bc, const double* p, int order,
+                           int ssp_stages, int use_limiters, int nsteps, const double* dts,
+                           const double* X0, const double* dX0, const double* A0,
+                           const double* dA0, int ngauge, const int* gauges, int obs_every,
+                           const double* obs, double* loss, double* dloss) {
+    const int nc = m->ncells;
+    std::vector<T> X((size_t)nc * NQ), A((size_t)nc * NA), P(M::n_parameters);
+    for (size_t i = 0; i < X.size(); ++i) {
+        X[i] = T(X0[i]);
+        X[i].d[0] = dX0 ? dX0[i] : 0.0;
+    }
+    for (size_t i = 0; i < A.size(); ++i) {
+        A[i] = T(A0[i]);
+        A[i].d[0] = dA0 ? dA0[i] : 0.0;
+    }
+    for (int i = 0; i < M::n_parameters; ++i) P[i] = T(p[i]);
+    const T L = run_loss(m, face_bc, P.data(), order, ssp_stages, use_limiters, nsteps, dts, X.data(),
+                         A.data(), ngauge, gauges, obs_every, obs);
+    *loss = L.v;
+    *dloss = L.d[0];
+    return 0;
+}
+#else
+extern "C" double orc_loss(const zdm_mesh* m, const int* face_bc, const double* p, int order,
+                           int ssp_stages, int use_limiters, int nsteps, const double* dts,
+                           const double* X0, const double* A0, int ngauge, const int* gauges,
+                           int obs_every, const double* obs) {
+    std::vector<T> X(X0, X0 + (size_t)m->ncells * NQ), A(A0, A0 + (size_t)m->ncells * NA);
+    return run_loss(m, face_bc, p, order, ssp_stages, use_limiters, nsteps, dts, X.data(), A.data(),
+                    ngauge, gauges, obs_every, obs);
+}
+#endif

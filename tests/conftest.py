@@ -199,6 +199,46 @@ class Oracle:
         return n, t[0], its.value, reason.value
 
 
+    # ---- calibration loss with frozen time steps (liboracle_<model>.so)
+    def loss(self, mesh, face_bc, p, X0, A0, dts, gauges, obs_every, obs, order=2, ssp_stages=2,
+             use_limiters=1):
+        L = self.L
+        L.orc_loss.restype = ctypes.c_double
+        L.orc_loss.argtypes = [ctypes.POINTER(zdm._ZdmMesh), _ip, _dp, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip,
+                               ctypes.c_int, _dp]
+        g = np.ascontiguousarray(gauges, dtype=np.int32)
+        return L.orc_loss(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters, len(dts),
+                          dp(np.ascontiguousarray(dts, dtype=np.float64)), dp(np.ascontiguousarray(X0)),
+                          dp(np.ascontiguousarray(A0)), len(g), ip(g), obs_every,
+                          dp(np.ascontiguousarray(obs, dtype=np.float64)))
+
+
+class TangentOracle:
+    """liboracle_<model>_tl.so: the oracle on one-direction dual numbers (tangent-linear run)."""
+
+    def __init__(self, model):
+        path = os.path.join(ORACLE_DIR, "liboracle_%s_tl.so" % model)
+        if not os.path.exists(path):
+            raise RuntimeError("tangent oracle not built: %s" % path)
+        L = ctypes.CDLL(path)
+        L.orc_tl_loss.argtypes = [ctypes.POINTER(zdm._ZdmMesh), _ip, _dp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                  _ip, ctypes.c_int, _dp, _dp, _dp]
+        self.L = L
+
+    def loss(self, mesh, face_bc, p, X0, dX0, A0, dA0, dts, gauges, obs_every, obs, order=2,
+             ssp_stages=2, use_limiters=1):
+        """(loss, directional derivative along (dX0, dA0))."""
+        g = np.ascontiguousarray(gauges, dtype=np.int32)
+        out, dout = np.zeros(1), np.zeros(1)
+        c = lambda a: dp(np.ascontiguousarray(a, dtype=np.float64)) if a is not None else None
+        self.L.orc_tl_loss(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters, len(dts),
+                           c(dts), c(X0), c(dX0), c(A0), c(dA0), len(g), ip(g), obs_every, c(obs),
+                           dp(out), dp(dout))
+        return out[0], dout[0]
+
+
 _oracles = {}
 
 

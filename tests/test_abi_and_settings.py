@@ -9,7 +9,7 @@ import pytest
 
 from conftest import ROOT, zdm
 
-MODELS = ["sme0_1d", "sme0", "sme1", "sme2"]
+MODELS = ["sme0_1d", "sme0", "sme1", "sme1m", "sme2", "vam"]
 
 
 def declared(header):

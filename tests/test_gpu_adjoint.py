@@ -1,0 +1,86 @@
+"""Discrete adjoint on the device against the tangent-linear oracle and finite differences.
+
+The gradient is the transpose of the derivative of the discrete step with frozen branches
+and time steps, so <gradient, v> must equal the tangent-linear oracle's directional
+derivative along v (same discrete map, forward mode); central differences of the device's
+own frozen-dt forward check it independently."""
+import numpy as np
+import pytest
+
+from conftest import TangentOracle, cases, oracle, zdm
+from test_oracle_tangent import calib_case
+
+pytestmark = pytest.mark.gpu
+
+MODEL = "sme1m"
+
+
+def make(order, steps=8, n=6, ck=0):
+    mesh, X, A, bc, gauges, dts, obs = calib_case(n=n, steps=steps)
+    s = zdm.Solver(MODEL, mesh, order=order, ssp_stages=2, cfl=0.45, t_end=1e9, use_limiters=True)
+    s.adjoint_setup(steps, gauges=gauges, obs_every=2, obs=obs, checkpoint_every=ck)
+    return s, mesh, X, A, bc, gauges, obs
+
+
+def directions(mesh, X, A, k, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        dX = np.zeros_like(X)
+        dX[:, 0] = 0.01 * rng.standard_normal(mesh.ncells)
+        dX[:, 1] = 0.01 * rng.standard_normal(mesh.ncells)
+        dX[:, 2] = 0.01 * rng.standard_normal(mesh.ncells)
+        dA = np.zeros_like(A)
+        dA[:, 1] = 0.005 * rng.standard_normal(mesh.ncells)
+        out.append((dX, dA))
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_adjoint_loss_and_gradient_vs_tangent_linear(order):
+    s, mesh, X, A, bc, gauges, obs = make(order)
+    print("case adjoint order", order, "cells", mesh.ncells, flush=True)
+    L = s.adjoint_forward(X, A)
+    dts = s.adjoint_dts()
+    assert np.all(dts > 0)
+    orc = oracle(MODEL)
+    Lo = orc.loss(mesh, bc, s.params, X, A, dts, gauges, 2, obs, order=order)
+    assert abs(L - Lo) <= 1e-13 * abs(Lo)
+    gX, gA = s.adjoint_gradient()
+    assert np.all(np.isfinite(gX)) and np.all(np.isfinite(gA))
+    tl = TangentOracle(MODEL)
+    for dX, dA in directions(mesh, X, A, 4):
+        _, dL = tl.loss(mesh, bc, s.params, X, dX, A, dA, dts, gauges, 2, obs, order=order)
+        ad = np.sum(gX * dX) + np.sum(gA * dA)
+        # the tangent run differentiates the Newton iterations (finite-difference aux Jacobian,
+        # stop at |R| < 1e-8); the adjoint transposes the converged implicit step
+        print("adjoint vs tangent-linear rel err", abs(ad - dL) / abs(dL))
+        assert abs(ad - dL) <= 1e-6 * abs(dL) + 1e-14, (ad, dL)
+    s.close()
+
+
+def test_adjoint_vs_central_differences_on_device():
+    s, mesh, X, A, bc, gauges, obs = make(2)
+    s.adjoint_forward(X, A)
+    dts = s.adjoint_dts()
+    gX, gA = s.adjoint_gradient()
+    eps = 1e-6
+    for dX, dA in directions(mesh, X, A, 3, seed=9):
+        Lp = s.adjoint_forward(X + eps * dX, A + eps * dA, dts=dts)
+        Lm = s.adjoint_forward(X - eps * dX, A - eps * dA, dts=dts)
+        fd = (Lp - Lm) / (2 * eps)
+        ad = np.sum(gX * dX) + np.sum(gA * dA)
+        print("adjoint vs central differences rel err", abs(ad - fd) / abs(fd))
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+    s.close()
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    grads = []
+    for ck in (1, 3, 8):
+        s, mesh, X, A, bc, gauges, obs = make(2, ck=ck)
+        s.adjoint_forward(X, A)
+        grads.append(s.adjoint_gradient())
+        s.close()
+    for g in grads[1:]:
+        assert np.array_equal(g[0], grads[0][0]) and np.array_equal(g[1], grads[0][1])

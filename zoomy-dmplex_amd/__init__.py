@@ -269,7 +269,8 @@ SOLVER_SYMBOLS = [
     "zdm_kernel_time", "zdm_stream_copy", "zdm_last_error",
     "zdm_chorin_setup", "zdm_chorin_step", "zdm_chorin_pressure", "zdm_vam_frozen_aux",
     "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres", "zdm_batch_create", "zdm_batch_time",
-    "zdm_step_graph",
+    "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
+    "zdm_adjoint_dts", "zdm_adjoint_residuals",
 ]
 
 
@@ -318,6 +319,12 @@ def model_lib(model):
                                      ctypes.POINTER(V)]
     lib.zdm_batch_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_int_p]
     lib.zdm_step_graph.argtypes = [V, ctypes.c_int]
+    lib.zdm_adjoint_setup.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      _c_int_p, _c_dbl_p]
+    lib.zdm_adjoint_forward.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+    lib.zdm_adjoint_gradient.argtypes = [V, _c_dbl_p, _c_dbl_p]
+    lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
+    lib.zdm_adjoint_residuals.argtypes = [V, _c_dbl_p]
     K = ctypes.POINTER(KrylovOptions)
     lib.zdm_chorin_setup.argtypes = [V, K]
     lib.zdm_chorin_step.argtypes = [V, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
@@ -502,6 +509,54 @@ class Solver:
         ms, n = ctypes.c_double(), ctypes.c_int()
         self._check(self.lib.zdm_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    # ---- discrete adjoint (calibration gradients)
+    def adjoint_setup(self, nsteps, gauges=(), obs_every=0, obs=None, checkpoint_every=0):
+        """Loss = sum over observation steps k*obs_every and gauges of (h - obs)^2."""
+        g = np.ascontiguousarray(gauges, dtype=np.int32)
+        nobs = nsteps // obs_every if len(g) and obs_every else 0
+        o = np.zeros((nobs, len(g))) if obs is None else np.ascontiguousarray(obs, dtype=np.float64)
+        if o.size != nobs * len(g):
+            raise ZdmError("adjoint_setup: obs must have nsteps // obs_every rows of len(gauges)")
+        self._adj = (int(nsteps), g, int(obs_every), o.reshape(nobs, len(g)))
+        self._check(self.lib.zdm_adjoint_setup(self._h, int(nsteps), int(checkpoint_every),
+                                               int(obs_every), len(g), g.ctypes.data_as(_c_int_p),
+                                               _dp(o) if o.size else None))
+
+    def adjoint_forward(self, X0, A0=None, dts=None):
+        """Forward run of the calibration problem; dts = None records the CFL steps."""
+        X0 = np.ascontiguousarray(X0, dtype=np.float64).reshape(self.mesh.ncells, self.info.n_q)
+        Ap = None
+        if A0 is not None:
+            A0 = np.ascontiguousarray(A0, dtype=np.float64).reshape(self.mesh.ncells, self.info.n_aux)
+            Ap = _dp(A0)
+        dp_ = None
+        if dts is not None:
+            dts = np.ascontiguousarray(dts, dtype=np.float64)
+            if dts.size != self._adj[0]:
+                raise ZdmError("adjoint_forward: dts must hold nsteps values")
+            dp_ = _dp(dts)
+        L = ctypes.c_double()
+        self._check(self.lib.zdm_adjoint_forward(self._h, _dp(X0), Ap, dp_, ctypes.byref(L)))
+        return L.value
+
+    def adjoint_gradient(self):
+        """(dL/dX0, dL/dA0) of the last adjoint_forward."""
+        gX = np.empty((self.mesh.ncells, self.info.n_q))
+        gA = np.empty((self.mesh.ncells, self.info.n_aux))
+        self._check(self.lib.zdm_adjoint_gradient(self._h, _dp(gX), _dp(gA)))
+        return gX, gA
+
+    def adjoint_dts(self):
+        d = np.empty(self._adj[0])
+        self._check(self.lib.zdm_adjoint_dts(self._h, _dp(d)))
+        return d
+
+    def adjoint_residuals(self):
+        r = np.empty(self._adj[3].shape)
+        if r.size:
+            self._check(self.lib.zdm_adjoint_residuals(self._h, _dp(r)))
+        return r
 
     # ---- Chorin pressure split (VAM models, ChorinVAMSolver, ChorinVAM.hpp:86-129)
     def chorin_setup(self, krylov=None, **kw):

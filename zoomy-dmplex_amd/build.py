@@ -15,7 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
-MODELS = ["sme0_1d", "sme0", "sme1", "sme2", "vam"]
+MODELS = ["sme0_1d", "sme0", "sme1", "sme1m", "sme2", "vam"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -47,6 +47,7 @@ def build_model(model, extra_src=None):
         src += extra_src
     tmp = out + ".tmp"
     _run([HIPCC] + HIP_FLAGS + ["-I" + INCLUDE, "-I" + mdir, "-I" + os.path.join(HERE, "csrc"),
+                                "-I" + os.path.join(HERE, "models"),
                                 '-DZDM_MODEL_NAME="%s"' % model] + src + ["-o", tmp])
     os.replace(tmp, out)
     return out

@@ -31,6 +31,7 @@
 #include "zdm.h"
 
 #pragma clang force_cuda_host_device begin
+#include "dual.H"   // before the model: its templates name std::sqrt / std::fabs on duals
 #include "Numerics.H"
 #ifdef ZOOMY_MODEL_IS_VAM
 #include "ChorinOps.H"
@@ -713,10 +714,13 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
 }
 
 #include "krylov.inc"
+#include "adjoint.inc"
 #ifdef ZOOMY_MODEL_IS_VAM
 #include "chorin.inc"
 #endif
 struct ChorinHost;
+struct AdjointHost;
+void adjoint_free(AdjointHost* ad);
 
 }  // namespace
 
@@ -743,6 +747,7 @@ struct zdm_solver {
     int nctrl;                        // control blocks: max(1, nmem)
     hipGraph_t graph;                 // one captured time step (zdm_step_graph)
     hipGraphExec_t graph_exec;
+    AdjointHost* adj;                 // calibration adjoint state (zdm_adjoint_*)
 };
 
 namespace {
@@ -1180,6 +1185,7 @@ int zdm_solver_destroy(zdm_solver* s) {
     if (s->graph) hipGraphDestroy(s->graph);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
     chorin_free(s->ch);
+    adjoint_free(s->adj);
     hipFree(s->arena);
     hipStreamDestroy(s->stream);
     delete s;
@@ -1684,5 +1690,43 @@ int zdm_vam_residual(zdm_solver*, double, const double*, double*) { return not_v
 int zdm_vam_pmat(zdm_solver*, double, int, int*, int*, double*) { return not_vam(); }
 
 #endif
+
+}  // extern "C"
+
+namespace {
+#include "adjoint_host.inc"
+}  // namespace
+
+extern "C" {
+
+int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
+                      const int* gauges, const double* obs) {
+    if (!s) return fail("null solver");
+    return adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs);
+}
+
+int zdm_adjoint_forward(zdm_solver* s, const double* X0, const double* A0, const double* dts, double* loss) {
+    if (!s || !X0) return fail("zdm_adjoint_forward: null argument");
+    if (!s->adj) return fail("zdm_adjoint_forward: call zdm_adjoint_setup first");
+    return adjoint_forward(s, X0, A0, dts, loss);
+}
+
+int zdm_adjoint_gradient(zdm_solver* s, double* gX0, double* gA0) {
+    if (!s) return fail("null solver");
+    if (!s->adj) return fail("zdm_adjoint_gradient: call zdm_adjoint_setup first");
+    return adjoint_reverse(s, gX0, gA0);
+}
+
+int zdm_adjoint_dts(zdm_solver* s, double* dts) {
+    if (!s || !s->adj || !dts) return fail("zdm_adjoint_dts: bad arguments");
+    std::copy(s->adj->dts.begin(), s->adj->dts.end(), dts);
+    return 0;
+}
+
+int zdm_adjoint_residuals(zdm_solver* s, double* r) {
+    if (!s || !s->adj || !r) return fail("zdm_adjoint_residuals: bad arguments");
+    std::copy(s->adj->res_h.begin(), s->adj->res_h.end(), r);
+    return 0;
+}
 
 }  // extern "C"
