@@ -253,17 +253,90 @@ def run_c4(zdm, cases, args):
     print(json.dumps(out))
 
 
+def run_c5(zdm, cases, args):
+    """Calibration gradient: forward run + adjoint sweep vs the plain forward run."""
+    n = args.grid if args.grid != 1000 else 354
+    mesh = zdm.Mesh.tri_grid(n, n, jitter=0.15, seed=5, renumber=args.order)
+    Xt, At, Xg, Ag, gauges = cases.c5_case(mesh)
+    T = args.steps
+    obs_every = 5
+    s = zdm.Solver("sme1m", mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e9, source=True,
+                   use_limiters=True)
+    # synthetic observations: h at the gauges every 5 steps of the hidden-truth run
+    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every)
+    s.adjoint_forward(Xt, At)
+    obs = s.adjoint_residuals().copy()
+    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs)
+    # plain forward of the first guess (the same steps, no recording)
+    s.upload(Xg, Ag)
+    s.initialize()
+    s.step(min(args.warmup, T))
+    s.synchronize()
+    s.upload(Xg, Ag)
+    s.initialize()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(T)
+    s.synchronize()
+    t_plain = time.perf_counter() - t0
+    s.adjoint_forward(Xg, Ag)          # warm-up of the recording forward and the sweep
+    s.adjoint_gradient()
+    t0 = time.perf_counter()
+    loss = s.adjoint_forward(Xg, Ag)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gX, gA = s.adjoint_gradient()
+    t_rev = time.perf_counter() - t0
+    if not (np.all(np.isfinite(gX)) and np.all(np.isfinite(gA))):
+        raise RuntimeError("non-finite gradient")
+    ncells = mesh.ncells
+    ratio = (t_fwd + t_rev) / t_plain
+    out = {
+        "metric": "adjoint/forward ratio (SME L1 + Manning calibration, FP64)",
+        "value": round(ratio, 3),
+        "unit": "x forward time",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_plain / T * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "SME level 1 + Manning, %dx%d grid jittered +-15%% (%d cells), %d SSP-RK2 "
+                        "steps, 32 gauges every %d steps, gradient wrt bed and Manning n"
+                        % (n, n, ncells, T, obs_every),
+            "cells": ncells,
+            "forward_s": round(t_plain, 4),
+            "forward_recording_s": round(t_fwd, 4),
+            "reverse_s": round(t_rev, 4),
+            "reverse_over_forward": round(t_rev / t_plain, 3),
+            "forward_mcell_steps_per_s": round(ncells * T / t_plain / 1e6, 2),
+            "loss": loss,
+            "grad_norm_bed": float(np.linalg.norm(gX[:, 0])),
+            "grad_norm_manning": float(np.linalg.norm(gA[:, 1])),
+            "checkpoint_every": int(np.ceil(np.sqrt(T))),
+        },
+        "roofline": None,
+    }
+    s.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: SME level 2 transport + source (headline); c3: VAM Chorin split; "
-                         "c4: batched vs serial ensemble")
+                         "c4: batched vs serial ensemble; c5: adjoint calibration gradient")
     ap.add_argument("--members", type=int, default=256, help="c4 ensemble size")
     ap.add_argument("--op-mode", type=int, default=0, choices=[0, 1],
                     help="c3 pressure operator: 0 matrix-free residual, 1 assembled Pmat")
     ap.add_argument("--max-it", type=int, default=400, help="c3 GMRES iteration cap")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: c2 200, c3 10, c4 300, c5 500)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--grid", type=int, default=1000)
     ap.add_argument("--order", default="rcm", choices=["hilbert", "rcm", "none"],
@@ -275,8 +348,12 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.steps is None:
+        args.steps = {"c2": 200, "c3": 10, "c4": 300, "c5": 500}[args.workload]
     zdm = load_package()
     import zoomy_dmplex_amd.cases as cases
+    if args.workload == "c5":
+        return run_c5(zdm, cases, args)
     if args.workload == "c3":
         return run_c3(zdm, cases, args)
     if args.workload == "c4":

@@ -177,12 +177,18 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
  * initial aux A0 (pass-through aux such as a per-cell friction coefficient).  The gradient
  * is the exact transpose of the discrete step as computed, with the branches of clamps,
  * bounds, limiter and wet/dry switches and the time steps frozen at the forward run.
- * Memory: a checkpoint every checkpoint_every steps (<= 0: ceil(sqrt(nsteps))) plus one
- * segment of states; the reverse sweep re-runs each segment once.  Order 1, or order 2 with
+ * Limiter: exact_limiter = 0 (the default for calibration) holds the limiter value alpha of
+ * every face fixed (the frozen-limiter linearisation: Barth-Jespersen's alpha is one minimum
+ * over all components, so its exact derivative couples them through 1/delta and grows without
+ * bound in quiet regions); exact_limiter = 1 differentiates alpha through its active branch
+ * (the exact derivative of the discrete step, for finite-difference checks).
+ * Memory: a checkpoint every checkpoint_every steps; <= 0 keeps every state when the
+ * trajectory fits in a quarter of the free device memory (no re-run), else ceil(sqrt(nsteps))
+ * checkpoints with one re-run of each segment.  Order 1, or order 2 with
  * the Green-Gauss gradient; not for ensembles or the well-balanced extension.  No
  * counterpart in the reference, which has no differentiation path. */
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
-                      const int* gauges, const double* obs);
+                      const int* gauges, const double* obs, int exact_limiter);
 /* Forward run (X0, A0 in the zdm_state_upload layout, A0 may be NULL).  dts = NULL takes
  * dt = CFL each step and records it; otherwise the given sequence is used (a frozen-dt
  * re-run, e.g. for finite differences).  Returns the loss. */

@@ -59,6 +59,7 @@ struct P3 {
     }
 };
 const T BIG = (T)std::numeric_limits<double>::max();
+bool g_freeze_limiter = false;   // tangent build: hold each face's limiter value fixed
 
 struct Opts {
     int order;            // 1 = PCM + forward Euler, 2 = MUSCL + SSP-RK
@@ -264,6 +265,9 @@ void reconstruct(const T* q, const T* grad, const double* xc, const double* xf, 
     T alpha = 1.;
     for (int i = 0; i < NQ; ++i)
         if (alpha > fin[i]) alpha = fin[i];
+#ifdef ORACLE_TANGENT
+    if (g_freeze_limiter) alpha = T(alpha.v);   // frozen-limiter linearisation
+#endif
     for (int i = 0; i < NQ; ++i) out[i] = q[i] + alpha * delta[i];
 }
 
@@ -733,8 +737,9 @@ extern "C" int orc_tl_loss(const zdm_mesh* m, const int* face_bc, const double* 
                            int ssp_stages, int use_limiters, int nsteps, const double* dts,
                            const double* X0, const double* dX0, const double* A0,
                            const double* dA0, int ngauge, const int* gauges, int obs_every,
-                           const double* obs, double* loss, double* dloss) {
+                           const double* obs, int freeze_limiter, double* loss, double* dloss) {
     const int nc = m->ncells;
+    g_freeze_limiter = freeze_limiter != 0;
     std::vector<T> X((size_t)nc * NQ), A((size_t)nc * NA), P(M::n_parameters);
     for (size_t i = 0; i < X.size(); ++i) {
         X[i] = T(X0[i]);

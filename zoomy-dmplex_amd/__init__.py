@@ -27,8 +27,30 @@ class ZdmError(RuntimeError):
     pass
 
 
+_torch_first = False
+
+
+def _bind_runtime():
+    """Load PyTorch (when installed) before any HIP library of this package.
+
+    PyTorch-ROCm ships its own HIP runtime with the same soname as the system's
+    (libamdhip64.so.7).  Loaded first, it also serves this package's libraries, so one HIP /
+    HSA runtime runs the process and torch tensors and these kernels share the device.  Loaded
+    after a library that pulled in the system runtime, torch would bring a second runtime that
+    finds no device.  ZDM_NO_TORCH=1 skips this (processes that never use torch)."""
+    global _torch_first
+    if _torch_first or os.environ.get("ZDM_NO_TORCH") == "1":
+        return
+    _torch_first = True
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def _load(name):
     # ZDM_LIB_DIR selects another build directory of the same libraries (e.g. a tuned build)
+    _bind_runtime()
     path = os.path.join(os.environ.get("ZDM_LIB_DIR", HERE), name)
     if not os.path.exists(path):
         raise ZdmError("%s is not built (run zoomy-dmplex_amd/build.py or __graft_entry__.build())"
@@ -320,7 +342,7 @@ def model_lib(model):
     lib.zdm_batch_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_int_p]
     lib.zdm_step_graph.argtypes = [V, ctypes.c_int]
     lib.zdm_adjoint_setup.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                      _c_int_p, _c_dbl_p]
+                                      _c_int_p, _c_dbl_p, ctypes.c_int]
     lib.zdm_adjoint_forward.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     lib.zdm_adjoint_gradient.argtypes = [V, _c_dbl_p, _c_dbl_p]
     lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
@@ -511,8 +533,11 @@ class Solver:
         return ms.value, n.value
 
     # ---- discrete adjoint (calibration gradients)
-    def adjoint_setup(self, nsteps, gauges=(), obs_every=0, obs=None, checkpoint_every=0):
-        """Loss = sum over observation steps k*obs_every and gauges of (h - obs)^2."""
+    def adjoint_setup(self, nsteps, gauges=(), obs_every=0, obs=None, checkpoint_every=0,
+                      exact_limiter=False):
+        """Loss = sum over observation steps k*obs_every and gauges of (h - obs)^2.
+        exact_limiter=False holds each face's limiter value fixed (frozen-limiter
+        linearisation); True differentiates it through its active branch."""
         g = np.ascontiguousarray(gauges, dtype=np.int32)
         nobs = nsteps // obs_every if len(g) and obs_every else 0
         o = np.zeros((nobs, len(g))) if obs is None else np.ascontiguousarray(obs, dtype=np.float64)
@@ -521,7 +546,7 @@ class Solver:
         self._adj = (int(nsteps), g, int(obs_every), o.reshape(nobs, len(g)))
         self._check(self.lib.zdm_adjoint_setup(self._h, int(nsteps), int(checkpoint_every),
                                                int(obs_every), len(g), g.ctypes.data_as(_c_int_p),
-                                               _dp(o) if o.size else None))
+                                               _dp(o) if o.size else None, int(bool(exact_limiter))))
 
     def adjoint_forward(self, X0, A0=None, dts=None):
         """Forward run of the calibration problem; dts = None records the CFL steps."""

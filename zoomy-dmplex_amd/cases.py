@@ -10,6 +10,8 @@ fed to the CPU oracle and to the GPU are identical and reproducible:
                  1000 x 1000 grid (2 000 000 cells), 16 random Gaussian bumps,
                  h = 1 - b + 0.1 * circular dam (r = 0.2), MUSCL + SSP-RK2, 500 steps
   C4  seed 4000+i ensemble member i (SME level 1, jittered grid, 8 bumps, random dam)
+  C5  seed 5     calibration: SME level 1 + Manning friction, smooth random Manning field
+                 n in [0.01, 0.05] and bed, dam break, h observed at 32 gauge cells
 
 The initial condition is evaluated at the cell centroids, as
 VirtualSolver::ProjectModelInitialCondition does (VirtualSolver.hpp:259-281).
@@ -129,3 +131,51 @@ def lake_at_rest(mesh, n_q, seed=7, eta=1.0, amp=0.3):
     xy = mesh.cell_centroid[:, :2]
     b = gaussian_bumps(xy, rng, 6, amp)
     return sme_state(n_q, b, eta - b)
+
+
+def smooth_field(xy, rng, modes=6):
+    """Smooth random field in [-1, 1]: normalised sum of random Fourier modes."""
+    f = np.zeros(xy.shape[0])
+    for _ in range(modes):
+        a = rng.uniform(-1.0, 1.0)
+        kx, ky = rng.uniform(0.5, 3.0), rng.uniform(0.5, 3.0)
+        ph = rng.uniform(0.0, 2.0 * math.pi)
+        f += a * np.sin(2.0 * math.pi * (kx * xy[:, 0] + ky * xy[:, 1]) + ph)
+    m = np.max(np.abs(f))
+    return f / m if m > 0 else f
+
+
+def c5_fields(mesh, seed):
+    """(bed, Manning n) smooth random fields: b in [0, 0.1] + a gentle slope, n in [0.01, 0.05]."""
+    rng = SplitMix64(seed)
+    xy = mesh.cell_centroid[:, :2]
+    b = 0.05 * (1.0 + smooth_field(xy, rng)) + 0.05 * xy[:, 0]
+    n = 0.03 + 0.02 * smooth_field(xy, rng)
+    return b, n
+
+
+def c5_case(mesh, seed=5, ngauge=32, x0=0.5, eta_hi=1.0, eta_lo=0.6):
+    """C5: truth run state/aux, first-guess state/aux and the gauge cells.
+    Dam break eta = eta_hi (x < x0) / eta_lo over the bed; the first guess has the bed and
+    Manning fields of another seed."""
+    rng = SplitMix64(seed + 1000)
+    xy = mesh.cell_centroid[:, :2]
+
+    def build(fseed):
+        b, n = c5_fields(mesh, fseed)
+        X = np.zeros((mesh.ncells, 6))
+        X[:, 0] = b
+        X[:, 1] = np.where(xy[:, 0] < x0, eta_hi, eta_lo) - b
+        A = np.zeros((mesh.ncells, 2))
+        A[:, 1] = n
+        return X, A
+
+    Xt, At = build(seed)
+    Xg, Ag = build(seed + 1)
+    gauges = []
+    while len(gauges) < ngauge:
+        g = rng.randint(0, mesh.ncells - 1)
+        if g not in gauges:
+            gauges.append(g)
+    return Xt, At, Xg, Ag, np.array(gauges, dtype=np.int32)
+

@@ -1700,9 +1700,11 @@ namespace {
 extern "C" {
 
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
-                      const int* gauges, const double* obs) {
+                      const int* gauges, const double* obs, int exact_limiter) {
     if (!s) return fail("null solver");
-    return adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs);
+    if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs)) return -1;
+    s->adj->freeze_limiter = exact_limiter ? 0 : 1;
+    return 0;
 }
 
 int zdm_adjoint_forward(zdm_solver* s, const double* X0, const double* A0, const double* dts, double* loss) {
