@@ -52,32 +52,70 @@ def step_bytes_per_cell(n_q, faces_per_cell, stages=2):
     return stages * (88 * n_q + 96 + 56 * faces_per_cell) + 32 * n_q + 8
 
 
-def cpu_baseline(zdm, cases, grid=250, steps=20):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from conftest import oracle  # CPU checker, timed here as the reported baseline
-    orc = oracle("sme2")
-    mesh = zdm.Mesh.tri_grid(grid, grid)
-    info = zdm.ModelInfo("sme2")
-    X = cases.c2_state(mesh, info.n_q, seed=2)
-    A = np.zeros((mesh.ncells, info.n_aux))
-    fb = zdm.boundary_face_bc(mesh, info.boundary_tags, info.dim)
-    p = info.default_params
-    # slope method: time T and 2T steps, report d(wall)/d(steps)
-    runs = []
-    for n in (steps, 2 * steps):
-        Xc, Ac = X.copy(), A.copy()
-        t0 = time.perf_counter()
-        orc.run(mesh, fb, p, Xc, Ac, order=2, ssp_stages=2, cfl=0.45, max_steps=n)
-        runs.append(time.perf_counter() - t0)
-    per_step = (runs[1] - runs[0]) / steps
+def pmc_traffic(kernel, cells):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE profile of the
+    same workload (profiles/k_flux_c2_2M_pmc.md, copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "k_flux_c2_2M_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("bench: "):
+                d = json.loads(line[len("bench: "):])
+                if d["kernel"] == kernel and d["cells"] == cells:
+                    return d["traffic_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
+_CPU_CHILD = r"""
+import sys, time, json
+sys.path.insert(0, sys.argv[1])
+from conftest import oracle, zdm, cases
+import numpy as np
+grid, steps = int(sys.argv[2]), int(sys.argv[3])
+orc = oracle("sme2")
+mesh = zdm.Mesh.tri_grid(grid, grid)
+info = zdm.ModelInfo("sme2")
+X = cases.c2_state(mesh, info.n_q, seed=2)
+A = np.zeros((mesh.ncells, info.n_aux))
+fb = zdm.boundary_face_bc(mesh, info.boundary_tags, info.dim)
+runs = []
+for n in (steps, 2 * steps):
+    Xc, Ac = X.copy(), A.copy()
+    t0 = time.perf_counter()
+    orc.run(mesh, fb, info.default_params, Xc, Ac, order=2, ssp_stages=2, cfl=0.45, max_steps=n)
+    runs.append(time.perf_counter() - t0)
+print(json.dumps({"cells": mesh.ncells, "per_step": (runs[1] - runs[0]) / steps}))
+"""
+
+
+def cpu_baseline(zdm, cases, grid=250, steps=20, procs=None):
+    """The oracle (C++ restatement of the reference step, one thread each) run as `procs`
+    concurrent processes on all host cores (at most 16), each on the same bounded sample; the
+    aggregate throughput is reported (slope of T vs 2T steps removes start-up)."""
+    import subprocess
+    procs = procs or max(1, min(16, os.cpu_count() or 1))
+    env = dict(os.environ, ZDM_NO_TORCH="1", OMP_NUM_THREADS="1")
+    cmd = [sys.executable, "-c", _CPU_CHILD, os.path.join(ROOT, "tests"), str(grid), str(steps)]
+    ps = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, text=True)
+          for _ in range(procs)]
+    res = []
+    for p in ps:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError("CPU baseline process failed: " + err[-2000:])
+        res.append(json.loads(out.strip().splitlines()[-1]))
+    cells = res[0]["cells"]
+    per_step = sum(r["per_step"] for r in res) / len(res)
     return {
-        "value": mesh.ncells / per_step / 1e6,
+        "value": procs * cells / per_step / 1e6,
         "unit": "Mcell-steps/s",
-        "cores": 1,
+        "cores": procs,
         "kind": "port",
-        "sample": "oracle (C++ restatement of the reference step, 1 thread) on the same "
-                  "workload at %dx%d (%d cells), slope of %d vs %d steps: %.1f ns/cell/step"
-                  % (grid, grid, mesh.ncells, steps, 2 * steps, per_step / mesh.ncells * 1e9),
+        "sample": "oracle (C++ restatement of the reference step, one thread per process) as %d "
+                  "concurrent processes, each on the same workload at %dx%d (%d cells), slope of %d "
+                  "vs %d steps: %.1f ns/cell/step per process"
+                  % (procs, grid, grid, cells, steps, 2 * steps, per_step / cells * 1e9),
     }
 
 
@@ -422,7 +460,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": pmc_traffic("k_flux<3, 2>", ncells),
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of the stage launch, copy-kernel "
+                              "calibrated (profiles/k_flux_c2_2M_pmc.md)",
             "bytes_per_cell": flux_pass_bytes_per_cell(info.n_q, F),
             "launches": flux_n,
         },
