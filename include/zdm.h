@@ -47,6 +47,9 @@ int zdm_model_default_params(double* params);              /* Model::default_par
 int zdm_model_boundary_tags(char* names64, int cap);
 /* Model::parameter_names() (Model.H:43), same layout. */
 int zdm_model_parameter_names(char* names64, int cap);
+/* Model::initial_condition(X, p) at one point (Model.H:264-274; optional in a generated model,
+ * VirtualSolver.hpp:33-38): n_q values into Q, or an error when the model emits none. */
+int zdm_model_initial_condition(const double* X, const double* params, double* Q);
 
 /* Create a solver on `mesh`.  face_bc[f] = model BC index of boundary face f, or -1 when its
  * tag maps to no model BC (such faces get no flux, TransportStep.hpp:498).  Replaces

@@ -53,11 +53,22 @@ def build_model(model, extra_src=None):
     return out
 
 
+def build_cli(model):
+    """zdm_cli_<model>: the C++ host (include/zoomy/) over libzdm_<model>.so, found next to it."""
+    out = os.path.join(HERE, "zdm_cli_%s" % model)
+    _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(HERE, "host", "main.cpp"),
+          "-L" + HERE, "-l:libzdm_%s.so" % model, "-l:libzdm_mesh.so", "-Wl,-rpath,$ORIGIN",
+          "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(models=None, jobs=4):
     models = models or MODELS
     outs = [build_mesh()]
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         outs += list(ex.map(build_model, models))
+    outs += [build_cli(m) for m in models]
     return outs
 
 

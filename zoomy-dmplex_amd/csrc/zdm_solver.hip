@@ -42,6 +42,14 @@
 #define ZDM_MODEL_NAME "model"
 #endif
 
+// Model::initial_condition(X, p) is optional (VirtualSolver.hpp:33-38)
+template <typename TT, typename = void>
+struct HasInitialCondition : std::false_type {};
+template <typename TT>
+struct HasInitialCondition<TT, std::void_t<decltype(Model<TT>::initial_condition(
+                                   std::declval<const TT*>(), std::declval<const TT*>()))>>
+    : std::true_type {};
+
 // Optional model hook (not emitted by the reference generator): centred bathymetry term of
 // the second-order hydrostatic reconstruction, added per face side when well_balanced is on.
 template <typename TT, typename = void>
@@ -907,6 +915,17 @@ int zdm_model_boundary_tags(char* names64, int cap) {
         std::strncpy(names64 + 64 * i, v[i].c_str(), 63);
     }
     return (int)v.size();
+}
+
+int zdm_model_initial_condition(const double* X, const double* params, double* Q) {
+    if (!X || !params || !Q) return fail("zdm_model_initial_condition: null argument");
+    if constexpr (HasInitialCondition<T>::value) {
+        auto q = M::initial_condition(X, params);
+        for (int i = 0; i < NQ; ++i) Q[i] = q[i];
+        return 0;
+    } else {
+        return fail("No io.initial_condition_file set and the model emits no initial_condition(x,p)");
+    }
 }
 
 int zdm_model_parameter_names(char* names64, int cap) {
