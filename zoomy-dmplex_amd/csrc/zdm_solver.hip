@@ -50,6 +50,32 @@ struct HasInitialCondition<TT, std::void_t<decltype(Model<TT>::initial_condition
                                    std::declval<const TT*>(), std::declval<const TT*>()))>>
     : std::true_type {};
 
+// Optional numerics hook (not emitted by the reference generator): numerical_flux and
+// numerical_fluctuations from one evaluation, with results identical to the two calls
+template <typename NN, typename S, typename = void>
+struct HasFusedFlux : std::false_type {};
+template <typename NN, typename S>
+struct HasFusedFlux<NN, S, std::void_t<decltype(NN::numerical_flux_fluctuations(
+                               std::declval<const S*>(), std::declval<const S*>(), std::declval<const S*>(),
+                               std::declval<const S*>(), std::declval<const S*>(), std::declval<const S*>(),
+                               std::declval<S*>(), std::declval<S*>()))>> : std::true_type {};
+
+// F = numerical_flux, D = numerical_fluctuations of one face (one evaluation when the numerics
+// provide the fused hook)
+template <typename NN, typename S>
+__host__ __device__ __forceinline__ void face_flux(const S* qL, const S* qR, const S* aL, const S* aR,
+                                                   const S* p, const S* n, S* F, S* D) {
+    constexpr int nq = Model<S>::n_dof_q;
+    if constexpr (HasFusedFlux<NN, S>::value) {
+        NN::numerical_flux_fluctuations(qL, qR, aL, aR, p, n, F, D);
+    } else {
+        auto f = NN::numerical_flux(qL, qR, aL, aR, p, n);
+        auto d = NN::numerical_fluctuations(qL, qR, aL, aR, p, n);
+        for (int i = 0; i < nq; ++i) F[i] = f[i];
+        for (int i = 0; i < 2 * nq; ++i) D[i] = d[i];
+    }
+}
+
 // Optional model hook (not emitted by the reference generator): centred bathymetry term of
 // the second-order hydrostatic reconstruction, added per face side when well_balanced is on.
 template <typename TT, typename = void>
@@ -407,8 +433,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
             const T* aRc = right ? ac : an;
             auto aL = M::update_aux_variables(qL, aLc, P.p, 0.0, xf);
             auto aR = M::update_aux_variables(qR, aRc, P.p, 0.0, xf);
-            auto F = NUM::numerical_flux(qL, qR, aL.data, aR.data, P.p, nh);
-            auto D = NUM::numerical_fluctuations(qL, qR, aL.data, aR.data, P.p, nh);
+            T F[NQ], D[2 * NQ];
+            face_flux<NUM, T>(qL, qR, aL.data, aR.data, P.p, nh, F, D);
             if (right) {
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
@@ -428,8 +454,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
                 auto aL = M::update_aux_variables(qo, ac, P.p, time, xf);
                 auto qR = M::boundary_conditions(bc, qo, aL.data, P.p, nh, xf, time, 0.0);
                 auto aR = M::update_aux_variables(qR.data, aL.data, P.p, time, xf);
-                auto F = NUM::numerical_flux(qo, qR.data, aL.data, aR.data, P.p, nh);
-                auto D = NUM::numerical_fluctuations(qo, qR.data, aL.data, aR.data, P.p, nh);
+                T F[NQ], D[2 * NQ];
+                face_flux<NUM, T>(qo, qR.data, aL.data, aR.data, P.p, nh, F, D);
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
                     fc[i] -= D[NQ + i] * area;
@@ -1580,8 +1606,8 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
             g.rtol = o.rtol;
             g.atol = o.atol;
             g.dtol = o.dtol;
-            auto op = [&](const T* xx, T* y) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 0); };
-            auto op_pc = [&](const T* xx, T* y) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 1); };
+            auto op = [&](const T* xx, T* y, const int* h) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 0, h); };
+            auto op_pc = [&](const T* xx, T* y, const int* h) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 1, h); };
             auto pc = [&](const T* bb, const T* t, T* z) { k_pbj_residual<<<cg, 256, 0, st>>>(nb, Dinv, bb, t, z); };
             const int r = gmres_solve(g, st, op_pc, op, pc, (const T*)(mem + oB), (T*)(mem + oX));
             if (r == -100 || hipGetLastError() != hipSuccess) {

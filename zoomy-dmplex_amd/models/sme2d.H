@@ -375,6 +375,68 @@ struct SmeNumerics2D {
         return res;
     }
 
+    // Both of the above from one pair of reconstructed states (optional hook, detected by the
+    // solver): the same operations in the same order as numerical_flux and
+    // numerical_fluctuations, so the results are identical to calling them separately.
+    PORTABLE_FN static inline void numerical_flux_fluctuations(const T* Q_minus, const T* Q_plus,
+                                                               const T* Qaux_minus, const T* Qaux_plus,
+                                                               const T* p, const T* n, T* Fout, T* Dout) {
+        const T bstar = std::max(Q_minus[0], Q_plus[0]);
+        HR sL, sR;
+        reconstruct(Q_minus, bstar, sL);
+        reconstruct(Q_plus, bstar, sR);
+        {
+            T FL[n_dof_q], FR[n_dof_q];
+            phys_flux(sL, p, n, FL);
+            phys_flux(sR, p, n, FR);
+            for (int i = 0; i < n_dof_q; ++i) Fout[i] = (T)0.5 * (FL[i] + FR[i]);
+        }
+        const T s = (T)0.5 * std::max(speed(sL, p, n), speed(sR, p, n));
+        T dq[n_dof_q];
+        dq[0] = 0;
+        dq[1] = sR.h - sL.h;
+        for (int k = 0; k < 2 * NM + NTR; ++k) dq[IU + k] = sR.m[k] - sL.m[k];
+        T nc[n_dof_q];
+        for (int i = 0; i < n_dof_q; ++i) nc[i] = 0;
+        if (NM > 1) {
+            const T gt[3] = {(T)0.1127016653792583, (T)0.5, (T)0.8872983346207417};
+            const T gw[3] = {(T)0.2777777777777778, (T)0.4444444444444444, (T)0.2777777777777778};
+            T ubar[2 * NM];
+            for (int k = 0; k < 2 * NM; ++k) ubar[k] = 0;
+            for (int q = 0; q < 3; ++q) {
+                const T h = sL.h + gt[q] * dq[1];
+                const T hi = M::hinv_kp(h, eps_flux);
+                for (int k = 0; k < 2 * NM; ++k) ubar[k] += gw[q] * ((sL.m[k] + gt[q] * dq[IU + k]) * hi);
+            }
+            T dD[NM];
+            for (int j = 0; j < NM; ++j) dD[j] = dq[IU + j] * n[0] + dq[IV + j] * n[1];
+            for (int i = 1; i < NM; ++i) {
+                T bx = -ubar[0] * dD[i], by = -ubar[NM] * dD[i];
+                for (int j = 1; j < NM; ++j)
+                    for (int k = 1; k < NM; ++k) {
+                        const T b = (T)K::B[(i * NM + j) * NM + k];
+                        if (b != (T)0) {
+                            bx += b * ubar[k] * dD[j];
+                            by += b * ubar[NM + k] * dD[j];
+                        }
+                    }
+                nc[IU + i] = bx;
+                nc[IV + i] = by;
+            }
+        }
+        const T gh = (T)0.5 * p[0];
+        for (int i = 0; i < n_dof_q; ++i) {
+            Dout[i] = (T)0.5 * nc[i] + s * dq[i];
+            Dout[n_dof_q + i] = (T)0.5 * nc[i] - s * dq[i];
+        }
+        const T cR = gh * (sR.h * sR.h - Q_plus[1] * Q_plus[1]);
+        const T cL = gh * (Q_minus[1] * Q_minus[1] - sL.h * sL.h);
+        Dout[IU] += n[0] * cR;
+        Dout[IV] += n[1] * cR;
+        Dout[n_dof_q + IU] += n[0] * cL;
+        Dout[n_dof_q + IV] += n[1] * cL;
+    }
+
     PORTABLE_FN static inline SimpleArray<T, 1> local_max_abs_eigenvalue(const T* Q, const T* Qaux,
                                                                           const T* p, const T* n) {
         SimpleArray<T, 1> res;
