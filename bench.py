@@ -304,7 +304,8 @@ def run_c5(zdm, cases, args):
     s.adjoint_setup(T, gauges=gauges, obs_every=obs_every)
     s.adjoint_forward(Xt, At)
     obs = s.adjoint_residuals().copy()
-    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs)
+    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs, checkpoint_every=args.checkpoint_every)
+    ck, tape = s.adjoint_layout()
     # plain forward of the first guess (the same steps, no recording)
     s.upload(Xg, Ag)
     s.initialize()
@@ -355,7 +356,8 @@ def run_c5(zdm, cases, args):
             "loss": loss,
             "grad_norm_bed": float(np.linalg.norm(gX[:, 0])),
             "grad_norm_manning": float(np.linalg.norm(gA[:, 1])),
-            "checkpoint_every": int(np.ceil(np.sqrt(T))),
+            "checkpoint_every": ck,
+            "tape": ["states or checkpoints", "stage states", "stage and face states"][tape],
         },
         "roofline": None,
     }
@@ -372,6 +374,9 @@ def main():
     ap.add_argument("--op-mode", type=int, default=0, choices=[0, 1],
                     help="c3 pressure operator: 0 matrix-free residual, 1 assembled Pmat")
     ap.add_argument("--max-it", type=int, default=400, help="c3 GMRES iteration cap")
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="c5 trajectory layout: -2/-1 tape with/without face states, 1 every "
+                         "state, k>1 checkpoints, 0 the richest that fits")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: c2 200, c3 10, c4 300, c5 500)")

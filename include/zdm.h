@@ -185,13 +185,22 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
  * over all components, so its exact derivative couples them through 1/delta and grows without
  * bound in quiet regions); exact_limiter = 1 differentiates alpha through its active branch
  * (the exact derivative of the discrete step, for finite-difference checks).
- * Memory: a checkpoint every checkpoint_every steps; <= 0 keeps every state when the
- * trajectory fits in a quarter of the free device memory (no re-run), else ceil(sqrt(nsteps))
- * checkpoints with one re-run of each segment.  Order 1, or order 2 with
+ * Memory (checkpoint_every):
+ *   -2  a tape of every step's state, stage states, post-stage states and face states: the
+ *       reverse sweep runs only transposed kernels (most memory);
+ *   -1  the same without the face states (the reverse sweep recomputes them per stage);
+ *    1  the state before every step (each step's stages are re-run in the reverse sweep);
+ *    k  a checkpoint every k steps (each segment is re-run once more);
+ *    0  the first of -2, -1 that fits in half of the free device memory, else 1 if the
+ *       states fit in a quarter of it, else k = ceil(sqrt(nsteps)).
+ * Every layout gives bitwise the same gradient.  Order 1, or order 2 with
  * the Green-Gauss gradient; not for ensembles or the well-balanced extension.  No
  * counterpart in the reference, which has no differentiation path. */
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
                       const int* gauges, const double* obs, int exact_limiter);
+/* The layout zdm_adjoint_setup chose: checkpoint interval (1 with a tape) and tape level
+ * (0 none, 1 stage states, 2 stage and face states). */
+int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape);
 /* Forward run (X0, A0 in the zdm_state_upload layout, A0 may be NULL).  dts = NULL takes
  * dt = CFL each step and records it; otherwise the given sequence is used (a frozen-dt
  * re-run, e.g. for finite differences).  Returns the loss. */

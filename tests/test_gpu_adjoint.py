@@ -77,13 +77,20 @@ def test_adjoint_vs_central_differences_on_device():
     s.close()
 
 
-def test_checkpoint_interval_does_not_change_the_gradient():
+@pytest.mark.parametrize("order", [1, 2])
+def test_trajectory_layout_does_not_change_the_gradient(order):
+    """Tape with face states (-2), tape of stage states (-1), every state (1), checkpoints
+    (3, 8) and the automatic choice (0): bitwise the same gradient, since what a tape keeps is
+    what the re-runs recompute."""
     grads = []
-    for ck in (0, 1, 3, 8):
-        s, mesh, X, A, bc, gauges, obs = make(2, ck=ck)
+    for ck, want in ((-2, (1, 2)), (-1, (1, 1)), (0, None), (1, (1, 0)), (3, (3, 0)), (8, (8, 0))):
+        s, mesh, X, A, bc, gauges, obs = make(order, ck=ck)
+        if want is not None:
+            assert s.adjoint_layout() == want
         s.adjoint_forward(X, A)
         grads.append(s.adjoint_gradient())
         s.close()
+    assert np.all(np.isfinite(grads[0][0])) and np.abs(grads[0][0]).max() > 0
     for g in grads[1:]:
         assert np.array_equal(g[0], grads[0][0]) and np.array_equal(g[1], grads[0][1])
 
@@ -115,4 +122,14 @@ def test_torch_autograd_function():
         fd = (ag.calibration_loss(s, Xp, Ap, dts) - ag.calibration_loss(s, Xm, Am, dts)) / (2 * eps)
     ad = float(torch.sum(X0.grad[:, 0] * db) + torch.sum(A0.grad[:, 1] * dn)) / 3.0
     assert abs(ad - float(fd)) <= 1e-6 * abs(float(fd))
+    s.close()
+
+
+def test_forward_that_reaches_t_end_early_is_an_error():
+    """Steps past t_end are no-ops of the forward with nothing recorded: refused."""
+    mesh, X, A, bc, gauges, dts, obs = calib_case(n=6, steps=8)
+    s = zdm.Solver(MODEL, mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e-9, use_limiters=True)
+    s.adjoint_setup(8, gauges=gauges, obs_every=2, obs=obs)
+    with pytest.raises(zdm.ZdmError, match="t_end"):
+        s.adjoint_forward(X, A)
     s.close()

@@ -292,7 +292,7 @@ SOLVER_SYMBOLS = [
     "zdm_chorin_setup", "zdm_chorin_step", "zdm_chorin_pressure", "zdm_vam_frozen_aux",
     "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres", "zdm_batch_create", "zdm_batch_time",
     "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
-    "zdm_adjoint_dts", "zdm_adjoint_residuals",
+    "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout",
 ]
 
 
@@ -347,6 +347,7 @@ def model_lib(model):
     lib.zdm_adjoint_gradient.argtypes = [V, _c_dbl_p, _c_dbl_p]
     lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_residuals.argtypes = [V, _c_dbl_p]
+    lib.zdm_adjoint_layout.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     K = ctypes.POINTER(KrylovOptions)
     lib.zdm_chorin_setup.argtypes = [V, K]
     lib.zdm_chorin_step.argtypes = [V, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
@@ -537,7 +538,9 @@ class Solver:
                       exact_limiter=False):
         """Loss = sum over observation steps k*obs_every and gauges of (h - obs)^2.
         exact_limiter=False holds each face's limiter value fixed (frozen-limiter
-        linearisation); True differentiates it through its active branch."""
+        linearisation); True differentiates it through its active branch.
+        checkpoint_every: -2 / -1 tape with / without face states, 1 every state, k > 1 a
+        checkpoint every k steps, 0 the richest layout that fits in device memory."""
         g = np.ascontiguousarray(gauges, dtype=np.int32)
         nobs = nsteps // obs_every if len(g) and obs_every else 0
         o = np.zeros((nobs, len(g))) if obs is None else np.ascontiguousarray(obs, dtype=np.float64)
@@ -547,6 +550,12 @@ class Solver:
         self._check(self.lib.zdm_adjoint_setup(self._h, int(nsteps), int(checkpoint_every),
                                                int(obs_every), len(g), g.ctypes.data_as(_c_int_p),
                                                _dp(o) if o.size else None, int(bool(exact_limiter))))
+
+    def adjoint_layout(self):
+        """(checkpoint_every, tape level) chosen by adjoint_setup."""
+        ck, tp = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.zdm_adjoint_layout(self._h, ctypes.byref(ck), ctypes.byref(tp)))
+        return ck.value, tp.value
 
     def adjoint_forward(self, X0, A0=None, dts=None):
         """Forward run of the calibration problem; dts = None records the CFL steps."""

@@ -76,6 +76,17 @@ __host__ __device__ __forceinline__ void face_flux(const S* qL, const S* qR, con
     }
 }
 
+// Optional numerics hook (not emitted by the reference generator): the reverse-mode derivative
+// of numerical_flux / numerical_fluctuations, gL += J_L^T (bF, bD), gR += J_R^T (bF, bD), for
+// numerics that read no aux; the adjoint uses it in place of forward-mode duals
+template <typename NN, typename S, typename = void>
+struct HasFluxVjp : std::false_type {};
+template <typename NN, typename S>
+struct HasFluxVjp<NN, S, std::void_t<decltype(NN::numerical_flux_fluctuations_vjp(
+                             std::declval<const S*>(), std::declval<const S*>(), std::declval<const S*>(),
+                             std::declval<const S*>(), std::declval<const S*>(), std::declval<const S*>(),
+                             std::declval<S*>(), std::declval<S*>()))>> : std::true_type {};
+
 // Optional model hook (not emitted by the reference generator): centred bathymetry term of
 // the second-order hydrostatic reconstruction, added per face side when well_balanced is on.
 template <typename TT, typename = void>
@@ -383,7 +394,7 @@ template <int MAXF, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 int stage, const T* __restrict__ QF,
                                                 const T* __restrict__ As, const T* In, T* Out,
-                                                const T* __restrict__ Ag, int wb) {
+                                                const T* __restrict__ Ag, int wb, const T* X0) {
     const Loc L = locate(m);
     ctrl += L.m;
     if (!ctrl->active) return;
@@ -490,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
         const T al = ctrl->al, be = ctrl->be, ga = ctrl->ga;
 #pragma unroll
         for (int i = 0; i < NQ; ++i)
-            Out[(size_t)i * ld + c] = al * In[(size_t)i * ld + c] + be * fc[i] + ga * Out[(size_t)i * ld + c];
+            Out[(size_t)i * ld + c] = al * In[(size_t)i * ld + c] + be * fc[i] + ga * X0[(size_t)i * ld + c];
     }
 }
 
@@ -668,6 +679,41 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
     }
 }
 
+// k_post<POST> (POST 1: constraints; POST 2: constraints, source, constraints) of a step whose
+// intermediate states are kept: reads the final stage's output X1, writes the source's solution
+// X3 and the aux A3 it saw (POST 2), the end-of-step state Xout and its aux A, and the CFL
+// partial minima; the same operations as k_post.  dt_log (may be null) receives the step's dt.
+template <int POST>
+__global__ __launch_bounds__(BLOCK) void k_post_record(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
+                                                       const T* __restrict__ X1, T* __restrict__ Xout, T* A,
+                                                       T* __restrict__ X3, T* __restrict__ A3,
+                                                       T* __restrict__ partials, T* __restrict__ dt_log) {
+    const Loc L = locate(m);
+    ctrl += L.m;
+    if (!ctrl->active) return;
+    if (dt_log && L.lb == 0 && threadIdx.x == 0) *dt_log = ctrl->dt;   // the step's dt
+    const int c = L.c;
+    const int ld = m.ld;
+    T ratio = DBL_MAX;
+    if (c < m.ncells) {
+        T q[NQ], a[NA];
+        load_soa<NQ>(X1, ld, c, q);
+        load_soa<NA>(A, ld, c, a);
+        update_state_cell(P, q, a);
+        if constexpr (POST == 2) {
+            store_soa<NA>(A3, ld, c, a);
+            newton_cell(P, ctrl->dt, q, a);
+            store_soa<NQ>(X3, ld, c, q);
+            update_state_cell(P, q, a);
+        }
+        store_soa<NQ>(Xout, ld, c, q);
+        store_soa<NA>(A, ld, c, a);
+        ratio = cfl_ratio(P, q, a, m.rc[c]);
+    }
+    ratio = block_min(ratio);
+    if (threadIdx.x == 0) partials[L.lb] = ratio;
+}
+
 // what: 0 = standalone ComputeTimeStep (-> dt_query), 1 = first dt, 2 = end of a step,
 //       3 = end of a Chorin step (next dt = max(CFL, min_dt), ChorinVAM.hpp:103)
 // One workgroup per ensemble member (one in all for a single mesh): the member's partial minima
@@ -817,53 +863,62 @@ bool chol_solve_host(int n, std::vector<T>& A, std::vector<T>& b) {
 }
 
 template <int MAXF>
-void launch_prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag) {
+void launch_prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* As, T* QF) {
     dim3 g(s->nblocks), b(BLOCK);
     const int lim = s->opt.use_limiters;
     if (s->opt.order < 2)
-        k_prep<MAXF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, s->As, s->QF, lim);
+        k_prep<MAXF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim);
     else if (s->opt.gradient == 1)
-        k_prep<MAXF, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, s->As, s->QF, lim);
+        k_prep<MAXF, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim);
     else
-        k_prep<MAXF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, s->As, s->QF, lim);
+        k_prep<MAXF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim);
 }
 
+// MODE_FINAL: Out = al In + be F(In) + ga X0 (X0 = Out in place)
 template <int MAXF>
-void launch_flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag) {
+void launch_flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag,
+                 const T* QF, const T* As, const T* X0) {
     dim3 g(s->nblocks), b(BLOCK);
     const int wb = s->opt.well_balanced;
     switch (mode) {
         case MODE_RHS:
-            k_flux<MAXF, MODE_RHS><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            k_flux<MAXF, MODE_RHS><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
         case MODE_EULER:
-            k_flux<MAXF, MODE_EULER><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            k_flux<MAXF, MODE_EULER><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
         case MODE_STAGE:
-            k_flux<MAXF, MODE_STAGE><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            k_flux<MAXF, MODE_STAGE><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
         default:
-            k_flux<MAXF, MODE_FINAL><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, s->QF, s->As, In, Out, Ag, wb);
+            k_flux<MAXF, MODE_FINAL><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
     }
 }
 
-void prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag) {
+// face states / clamped aux into (QF, As): the solver's own buffers unless given
+void prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* QF = nullptr, T* As = nullptr) {
+    QF = QF ? QF : s->QF;
+    As = As ? As : s->As;
     if (s->maxf <= 2)
-        launch_prep<2>(s, ctrl, Xin, Ag);
+        launch_prep<2>(s, ctrl, Xin, Ag, As, QF);
     else if (s->maxf == 3)
-        launch_prep<3>(s, ctrl, Xin, Ag);
+        launch_prep<3>(s, ctrl, Xin, Ag, As, QF);
     else
-        launch_prep<4>(s, ctrl, Xin, Ag);
+        launch_prep<4>(s, ctrl, Xin, Ag, As, QF);
 }
 
-void flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag) {
+void flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag,
+          const T* QF = nullptr, const T* As = nullptr, const T* X0 = nullptr) {
+    QF = QF ? QF : s->QF;
+    As = As ? As : s->As;
+    X0 = X0 ? X0 : Out;
     if (s->maxf <= 2)
-        launch_flux<2>(s, ctrl, mode, stage, In, Out, Ag);
+        launch_flux<2>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
     else if (s->maxf == 3)
-        launch_flux<3>(s, ctrl, mode, stage, In, Out, Ag);
+        launch_flux<3>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
     else
-        launch_flux<4>(s, ctrl, mode, stage, In, Out, Ag);
+        launch_flux<4>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
 }
 
 void post(zdm_solver* s, int mode, T dt_override = 0) {
@@ -1749,6 +1804,13 @@ int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_e
     if (!s) return fail("null solver");
     if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs)) return -1;
     s->adj->freeze_limiter = exact_limiter ? 0 : 1;
+    return 0;
+}
+
+int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape) {
+    if (!s || !s->adj) return fail("zdm_adjoint_layout: call zdm_adjoint_setup first");
+    if (checkpoint_every) *checkpoint_every = s->adj->ck;
+    if (tape) *tape = s->adj->tape;
     return 0;
 }
 

@@ -437,6 +437,226 @@ struct SmeNumerics2D {
         Dout[n_dof_q + IV] += n[1] * cL;
     }
 
+    // ---- reverse-mode derivative of numerical_flux_fluctuations (optional hook) ----
+    // Branches of max / fabs are taken where the forward takes them: std::max(a, b) follows b
+    // when a < b, fabs(a) has slope -1 when a < 0 (the choices forward-mode duals make through
+    // the functions above), so the result is the exact transpose of their Jacobian.
+
+    // d hinv_kp(h, eps) / dh times bres
+    PORTABLE_FN static inline T hinv_kp_vjp(T h, T eps, T bres) {
+        const T hm = std::max(eps, h);
+        const T h2 = h * h, hm2 = hm * hm;
+        const T den = std::sqrt(h2 * h2 + hm2 * hm2);
+        const T c = (T)1.4142135623730951;
+        const T bnum = c * bres / den;
+        const T bden = -(c * std::max((T)0, h)) * bres / (den * den);
+        const T barg = bden / ((T)2 * den);
+        T bh = (T)2 * h * ((T)2 * h2 * barg);
+        if (eps < h) bh += (T)2 * hm * ((T)2 * hm2 * barg);
+        if ((T)0 < h) bh += bnum;
+        return bh;
+    }
+
+    // adjoint of phys_flux: bF -> bh, bm, bu (accumulated)
+    PORTABLE_FN static inline void phys_flux_vjp(const HR& s, const T* p, const T* n, const T* bF, T& bh, T* bm,
+                                                 T* bu) {
+        const T* u = s.u;
+        const T* m = s.m;
+        T an[NM], ban[NM];
+        for (int k = 0; k < NM; ++k) {
+            an[k] = u[k] * n[0] + u[NM + k] * n[1];
+            ban[k] = 0;
+        }
+        const T un = an[0];
+        T bun = m[0] * bF[IU] + m[NM] * bF[IV];
+        bm[0] += n[0] * bF[1] + un * bF[IU];
+        bm[NM] += n[1] * bF[1] + un * bF[IV];
+        for (int j = 1; j < NM; ++j) {
+            const T a = (T)K::A[j * NM + j];
+            bm[j] += a * an[j] * bF[IU];
+            bm[NM + j] += a * an[j] * bF[IV];
+            ban[j] += a * (m[j] * bF[IU] + m[NM + j] * bF[IV]);
+        }
+        bh += p[0] * s.h * (n[0] * bF[IU] + n[1] * bF[IV]);   // pr = g h^2 / 2
+        for (int i = 1; i < NM; ++i) {
+            const T fx = bF[IU + i], fy = bF[IV + i];
+            bh += an[i] * (u[0] * fx + u[NM] * fy);
+            bu[0] += s.h * an[i] * fx;
+            bu[NM] += s.h * an[i] * fy;
+            ban[i] += s.h * (u[0] * fx + u[NM] * fy);
+            bm[i] += un * fx;
+            bm[NM + i] += un * fy;
+            bun += m[i] * fx + m[NM + i] * fy;
+            for (int j = 1; j < NM; ++j)
+                for (int k = 1; k < NM; ++k) {
+                    const T a = (T)K::A[(i * NM + j) * NM + k];
+                    if (a != (T)0) {
+                        bm[j] += a * an[k] * fx;
+                        bm[NM + j] += a * an[k] * fy;
+                        ban[k] += a * (m[j] * fx + m[NM + j] * fy);
+                    }
+                }
+        }
+        for (int k = 0; k < NTR; ++k) {
+            bm[2 * NM + k] += un * bF[IT + k];
+            bun += m[2 * NM + k] * bF[IT + k];
+        }
+        ban[0] += bun;
+        for (int k = 0; k < NM; ++k) {
+            bu[k] += n[0] * ban[k];
+            bu[NM + k] += n[1] * ban[k];
+        }
+    }
+
+    // adjoint of speed: bv -> bh, bu (accumulated)
+    PORTABLE_FN static inline void speed_vjp(const HR& s, const T* p, const T* n, T bv, T& bh, T* bu) {
+        for (int j = 0; j < NM; ++j) {
+            const T a = s.u[j] * n[0] + s.u[NM + j] * n[1];
+            const T sb = a < (T)0 ? -bv : bv;
+            bu[j] += n[0] * sb;
+            bu[NM + j] += n[1] * sb;
+        }
+        bh += p[0] * bv / ((T)2 * std::sqrt(p[0] * s.h));
+    }
+
+    // adjoint of reconstruct: (bh, bm, bu) -> g (accumulated); returns the adjoint of bstar
+    PORTABLE_FN static inline T reconstruct_vjp(const T* Q, T bstar, const HR& s, T bh, T* bm, const T* bu,
+                                                T* g) {
+        const T den = std::max((T)1.0e-14, Q[1]);
+        const T r = s.h / den;
+        const T hi = M::hinv_kp(s.h, eps_flux);
+        T bhi = 0;
+        for (int k = 0; k < 2 * NM; ++k) {
+            bm[k] += hi * bu[k];
+            bhi += s.m[k] * bu[k];
+        }
+        bh += hinv_kp_vjp(s.h, eps_flux, bhi);
+        T br = 0;
+        for (int k = 0; k < 2 * NM; ++k) {
+            g[IU + k] += r * bm[k];
+            br += Q[IU + k] * bm[k];
+        }
+        for (int k = 0; k < NTR; ++k) {
+            g[IT + k] += r * bm[2 * NM + k];
+            br += Q[IT + k] * bm[2 * NM + k];
+        }
+        bh += br / den;
+        if ((T)1.0e-14 < Q[1]) g[1] -= r * br / den;
+        if ((T)0 < Q[0] + Q[1] - bstar) {
+            g[0] += bh;
+            g[1] += bh;
+            return -bh;
+        }
+        return (T)0;
+    }
+
+    // gL += (d Phi / d Q_minus)^T, gR += (d Phi / d Q_plus)^T for Phi = bF . F + bD . D, with
+    // (F, D) = numerical_flux_fluctuations(Q_minus, Q_plus, ...).  These numerics read neither
+    // aux argument, so the states are the only inputs.
+    PORTABLE_FN static inline void numerical_flux_fluctuations_vjp(const T* Q_minus, const T* Q_plus, const T* p,
+                                                                   const T* n, const T* bF, const T* bD, T* gL,
+                                                                   T* gR) {
+        constexpr int NMOM = 2 * NM + NTR;
+        const T bstar = std::max(Q_minus[0], Q_plus[0]);
+        HR sL, sR;
+        reconstruct(Q_minus, bstar, sL);
+        reconstruct(Q_plus, bstar, sR);
+        T bhL = 0, bhR = 0, bmL[NMOM], bmR[NMOM], buL[2 * NM], buR[2 * NM];
+        for (int k = 0; k < NMOM; ++k) bmL[k] = bmR[k] = 0;
+        for (int k = 0; k < 2 * NM; ++k) buL[k] = buR[k] = 0;
+        {   // F = (F(q*_L) + F(q*_R)) / 2
+            T hb[n_dof_q];
+            for (int i = 0; i < n_dof_q; ++i) hb[i] = (T)0.5 * bF[i];
+            phys_flux_vjp(sL, p, n, hb, bhL, bmL, buL);
+            phys_flux_vjp(sR, p, n, hb, bhR, bmR, buR);
+        }
+        const T spL = speed(sL, p, n), spR = speed(sR, p, n);
+        const T sp = (T)0.5 * std::max(spL, spR);
+        T dq[n_dof_q], bdq[n_dof_q];
+        dq[0] = 0;
+        dq[1] = sR.h - sL.h;
+        for (int k = 0; k < NMOM; ++k) dq[IU + k] = sR.m[k] - sL.m[k];
+        T bsp = 0;
+        for (int i = 0; i < n_dof_q; ++i) {
+            const T t = bD[i] - bD[n_dof_q + i];
+            bsp += dq[i] * t;
+            bdq[i] = sp * t;
+        }
+        if (NM > 1) {
+            const T gt[3] = {(T)0.1127016653792583, (T)0.5, (T)0.8872983346207417};
+            const T gw[3] = {(T)0.2777777777777778, (T)0.4444444444444444, (T)0.2777777777777778};
+            T hq[3], hiq[3], ubar[2 * NM], bubar[2 * NM];
+            for (int k = 0; k < 2 * NM; ++k) ubar[k] = bubar[k] = 0;
+            for (int q = 0; q < 3; ++q) {
+                hq[q] = sL.h + gt[q] * dq[1];
+                hiq[q] = M::hinv_kp(hq[q], eps_flux);
+                for (int k = 0; k < 2 * NM; ++k) ubar[k] += gw[q] * ((sL.m[k] + gt[q] * dq[IU + k]) * hiq[q]);
+            }
+            T dD[NM], bdD[NM];
+            for (int j = 0; j < NM; ++j) {
+                dD[j] = dq[IU + j] * n[0] + dq[IV + j] * n[1];
+                bdD[j] = 0;
+            }
+            for (int i = 1; i < NM; ++i) {
+                const T bx = (T)0.5 * (bD[IU + i] + bD[n_dof_q + IU + i]);
+                const T by = (T)0.5 * (bD[IV + i] + bD[n_dof_q + IV + i]);
+                bubar[0] -= dD[i] * bx;
+                bubar[NM] -= dD[i] * by;
+                bdD[i] -= ubar[0] * bx + ubar[NM] * by;
+                for (int j = 1; j < NM; ++j)
+                    for (int k = 1; k < NM; ++k) {
+                        const T b = (T)K::B[(i * NM + j) * NM + k];
+                        if (b != (T)0) {
+                            bubar[k] += b * dD[j] * bx;
+                            bubar[NM + k] += b * dD[j] * by;
+                            bdD[j] += b * (ubar[k] * bx + ubar[NM + k] * by);
+                        }
+                    }
+            }
+            for (int j = 0; j < NM; ++j) {
+                bdq[IU + j] += n[0] * bdD[j];
+                bdq[IV + j] += n[1] * bdD[j];
+            }
+            for (int q = 0; q < 3; ++q) {
+                T bhi = 0;
+                for (int k = 0; k < 2 * NM; ++k) {
+                    const T w = gw[q] * bubar[k];
+                    bmL[k] += hiq[q] * w;
+                    bdq[IU + k] += gt[q] * (hiq[q] * w);
+                    bhi += (sL.m[k] + gt[q] * dq[IU + k]) * w;
+                }
+                const T bh = hinv_kp_vjp(hq[q], eps_flux, bhi);
+                bhL += bh;
+                bdq[1] += gt[q] * bh;
+            }
+        }
+        {   // hydrostatic-reconstruction pressure corrections
+            const T gh = (T)0.5 * p[0];
+            const T bcR = n[0] * bD[IU] + n[1] * bD[IV];
+            const T bcL = n[0] * bD[n_dof_q + IU] + n[1] * bD[n_dof_q + IV];
+            bhR += gh * ((T)2 * sR.h) * bcR;
+            gR[1] -= gh * ((T)2 * Q_plus[1]) * bcR;
+            gL[1] += gh * ((T)2 * Q_minus[1]) * bcL;
+            bhL -= gh * ((T)2 * sL.h) * bcL;
+        }
+        if (spL < spR)
+            speed_vjp(sR, p, n, (T)0.5 * bsp, bhR, buR);
+        else
+            speed_vjp(sL, p, n, (T)0.5 * bsp, bhL, buL);
+        bhR += bdq[1];
+        bhL -= bdq[1];
+        for (int k = 0; k < NMOM; ++k) {
+            bmR[k] += bdq[IU + k];
+            bmL[k] -= bdq[IU + k];
+        }
+        const T bb = reconstruct_vjp(Q_minus, bstar, sL, bhL, bmL, buL, gL) +
+                     reconstruct_vjp(Q_plus, bstar, sR, bhR, bmR, buR, gR);
+        if (Q_minus[0] < Q_plus[0])
+            gR[0] += bb;
+        else
+            gL[0] += bb;
+    }
+
     PORTABLE_FN static inline SimpleArray<T, 1> local_max_abs_eigenvalue(const T* Q, const T* Qaux,
                                                                           const T* p, const T* n) {
         SimpleArray<T, 1> res;
