@@ -37,6 +37,11 @@ typedef struct zdm_options {
     int block_size;     /* workgroup size (0 = 256) */
     int well_balanced;  /* extension: add Model::hydrostatic_cell_source per face side (second-order
                            hydrostatic reconstruction); 0 = the reference scheme */
+    int positivity;     /* solver.positivity: 0 "none", 1 "mood" (order 2: eta-WB reconstruction,
+                           ModularSolver.hpp:46-58; any order: the local MOOD override of cells with
+                           h < -1e-10, MUSCLSolver.hpp:130-171), 2 "zhang_shu" (order 2: eta-WB +
+                           Zhang-Shu theta, CFL capped at 1/6, MUSCLSolver.hpp:31-38) */
+    double wet_dry_eps; /* solver.wet_dry_eps (<= 0: the default 1e-2) */
 } zdm_options;
 
 /* Model plug-in description (Model.H:31-44). */
@@ -198,6 +203,9 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
  * counterpart in the reference, which has no differentiation path. */
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
                       const int* gauges, const double* obs, int exact_limiter);
+/* MOOD overrides so far (cells of h < -1e-10 replaced by their first-order update), summed
+ * over ensemble members; reset by zdm_initialize.  The reference prints this count per step. */
+int zdm_troubled_cells(zdm_solver* s, long long* count);
 /* The layout zdm_adjoint_setup chose: checkpoint interval (1 with a tape) and tape level
  * (0 none, 1 stage states, 2 stage and face states). */
 int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape);

@@ -275,6 +275,15 @@ public:
     explicit LinearReconstructor(bool with_limiters) { this->limited = with_limiters; }
     int Order() const override { return 2; }
 };
+// WBPositivityReconstructor (Reconstruction.hpp:246-287): eta-WB faces from the effective
+// gradients of TransportStep::LimitGradientsWB (positivity "mood" / "zhang_shu" at order 2)
+template <typename T>
+class WBPositivityReconstructor : public Reconstructor<T> {
+public:
+    explicit WBPositivityReconstructor(T eps) : eps_wet(eps) { this->limited = true; }
+    int Order() const override { return 2; }
+    T eps_wet;
+};
 
 // ---------------------------------------------------------------- the step components
 template <typename T>
@@ -458,9 +467,13 @@ public:
                 return Fail("time_integration '" + ss.time_integration + "' is not provided by this build "
                             "(splitting only)");
         }
-        if (ss.positivity != "none" && verbose)
-            std::cout << "[WARN] positivity '" << ss.positivity << "' is not provided by this build; "
-                      << "running the plain limited scheme" << std::endl;
+        int positivity = 0;
+        if (ss.positivity == "mood")
+            positivity = 1;
+        else if (ss.positivity == "zhang_shu")
+            positivity = 2;
+        else if (ss.positivity != "none")
+            return Fail("positivity '" + ss.positivity + "' is not one of none, mood, zhang_shu");
         zdm_options o;
         std::memset(&o, 0, sizeof(o));
         o.order = config_reconstruction_order;
@@ -472,12 +485,17 @@ public:
         o.min_dt = ss.min_dt;
         o.block_size = 0;
         o.well_balanced = ss.well_balanced ? 1 : 0;
+        o.positivity = positivity;
+        o.wet_dry_eps = ss.wet_dry_eps;
         ZOOMY_TRY(strategy->SetupTS(this, o));
         if (h_) zdm_solver_destroy(h_);
         h_ = nullptr;
         ZOOMY_CALL(zdm_solver_create(mesh.get(), face_bc.data(), parameters.data(), &o, &h_));
         transport = std::make_unique<TransportStep<double>>(h_, info);
-        if (o.order >= 2) {
+        if (o.order >= 2 && positivity != 0) {
+            transport->SetReconstruction(std::make_shared<WBPositivityReconstructor<double>>(ss.wet_dry_eps));
+            transport->SetGradient(std::make_shared<GreenGaussGradient<double>>());
+        } else if (o.order >= 2) {
             transport->SetReconstruction(std::make_shared<LinearReconstructor<double>>(config_use_limiters));
             if (o.gradient)
                 transport->SetGradient(std::make_shared<LeastSquaresGradient<double>>());
@@ -522,6 +540,11 @@ public:
         ZOOMY_CALL(zdm_run(h_, max_steps, 16, &done));
         double dt = 0;
         ZOOMY_CALL(zdm_get_time(h_, &time, &dt, &steps));
+        if (settings.solver.positivity == "mood") {
+            long long nt = 0;
+            ZOOMY_CALL(zdm_troubled_cells(h_, &nt));
+            if (verbose) std::cout << "[MOOD] troubled cells overridden: " << nt << std::endl;
+        }
         if (verbose) std::cout << "[INFO] Finished: " << steps << " steps, time " << time << std::endl;
         return 0;
     }

@@ -15,6 +15,7 @@
 // service's ordering).
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -60,6 +61,13 @@ struct P3 {
 };
 const T BIG = (T)std::numeric_limits<double>::max();
 bool g_freeze_limiter = false;   // tangent build: hold each face's limiter value fixed
+// solver.positivity at order 2: 0 none, 1 "mood" (eta-WB reconstruction, no theta, MOOD
+// override), 2 "zhang_shu" (eta-WB reconstruction with the Zhang-Shu theta); wet_dry_eps
+int g_positivity = 0;
+double g_wet_eps = 1e-2;
+long long g_troubled = 0;                    // MOOD: cells overridden so far
+std::vector<uint8_t> g_last_mask;            // MOOD: the cells overridden in the last step that had any
+const std::vector<uint8_t>* g_filter = nullptr;   // MOOD: faces touching these cells only
 
 struct Opts {
     int order;            // 1 = PCM + forward Euler, 2 = MUSCL + SSP-RK
@@ -271,6 +279,125 @@ void reconstruct(const T* q, const T* grad, const double* xc, const double* xf, 
     for (int i = 0; i < NQ; ++i) out[i] = q[i] + alpha * delta[i];
 }
 
+// TransportStep::LimitGradientsWB, TransportStep.hpp:241-323: the raw gradient G (of Q = [b,
+// h, mom...]) becomes the effective W-gradients in place: slot b -> phi_b grad b, slot h ->
+// phi_b grad b + theta sh, mom -> theta phi_mom grad mom; phi = Venkatakrishnan per W variable
+// (W = b, eta = b + h, mom) over the face-neighbour extrema of the clamped states Xl, eps^2 =
+// vol^(2/dim) (vol itself in 2-D; vol*vol in 1-D, where the reference's pow(vol, 2.0) may round
+// differently in the last bit); theta = Zhang-Shu cap unless MOOD (skip_theta).
+void limit_gradients_wb(const zdm_mesh* m, const T* Xl, T* G, bool skip_theta, double eps_wet) {
+    constexpr int B = 0, H = 1;
+    for (int c = 0; c < m->ncells; ++c) {
+        T* gc = G + (size_t)c * NQ * DIM;
+        const T* qc = Xl + (size_t)c * NQ;
+        const double* cc = m->cell_centroid + 3 * (size_t)c;
+        T g[NQ * DIM];
+        for (int k = 0; k < NQ * DIM; ++k) g[k] = gc[k];
+        T geta[DIM];
+        for (int d = 0; d < DIM; ++d) geta[d] = g[H * DIM + d] + g[B * DIM + d];
+        const T h_bar = qc[H], b_bar = qc[B];
+        T Wc[NQ], Wmin[NQ], Wmax[NQ];
+        for (int i = 0; i < NQ; ++i) Wc[i] = (i == H) ? (b_bar + h_bar) : qc[i];
+        for (int i = 0; i < NQ; ++i) {
+            Wmin[i] = Wc[i];
+            Wmax[i] = Wc[i];
+        }
+        for (int k = 0; k < m->cell_nfaces[c]; ++k) {
+            const int n = m->cell_nbrs[(size_t)c * m->max_faces + k];
+            if (n < 0) continue;
+            const T* qn = Xl + (size_t)n * NQ;
+            for (int i = 0; i < NQ; ++i) {
+                T w = (i == H) ? (qn[B] + qn[H]) : qn[i];
+                if (w < Wmin[i]) Wmin[i] = w;
+                if (w > Wmax[i]) Wmax[i] = w;
+            }
+        }
+        const T vol = m->cell_volume[c];
+        const T eps2 = DIM == 2 ? vol : vol * vol;
+        T phi[NQ];
+        for (int i = 0; i < NQ; ++i) phi[i] = 1.0;
+        auto gW = [&](int i, int d) { return (i == H) ? geta[d] : g[i * DIM + d]; };
+        for (int k = 0; k < m->cell_nfaces[c]; ++k) {
+            const int f = m->cell_faces[(size_t)c * m->max_faces + k];
+            const double* xf = m->face_centroid + 3 * (size_t)f;
+            T r[3];
+            for (int d = 0; d < DIM; ++d) r[d] = xf[d] - cc[d];
+            for (int i = 0; i < NQ; ++i) {
+                T delta = 0;
+                for (int d = 0; d < DIM; ++d) delta += gW(i, d) * r[d];
+                if (std::fabs(delta) < 1e-14) continue;
+                T dm = (delta > 0) ? (Wmax[i] - Wc[i]) : (Wmin[i] - Wc[i]);
+                T num = dm * dm + eps2 + 2 * delta * dm, den = dm * dm + 2 * delta * delta + delta * dm + eps2;
+                T pf = (den > 1e-30) ? num / den : (T)1.0;
+                pf = std::max((T)0, std::min((T)1, pf));
+                if (pf < phi[i]) phi[i] = pf;
+            }
+        }
+        if (h_bar < eps_wet)
+            for (int i = 0; i < NQ; ++i) phi[i] = 0.0;
+        phi[B] = std::min(phi[B], phi[H]);
+        for (int i = 0; i < NQ; ++i)
+            if (i != B && i != H) phi[i] = std::min(phi[i], phi[H]);
+        T sh[DIM];
+        for (int d = 0; d < DIM; ++d) sh[d] = phi[H] * geta[d] - phi[B] * g[B * DIM + d];
+        T hmin = h_bar;
+        for (int k = 0; k < m->cell_nfaces[c]; ++k) {
+            const int f = m->cell_faces[(size_t)c * m->max_faces + k];
+            const double* xf = m->face_centroid + 3 * (size_t)f;
+            T hf = h_bar;
+            for (int d = 0; d < DIM; ++d) hf += sh[d] * (xf[d] - cc[d]);
+            if (hf < hmin) hmin = hf;
+        }
+        T theta = 1.0;
+        if (!skip_theta && hmin < 0.0)
+            theta = std::max((T)0, std::min((T)1, h_bar / std::max(h_bar - hmin, (T)1e-14)));
+        for (int d = 0; d < DIM; ++d) {
+            gc[B * DIM + d] = phi[B] * g[B * DIM + d];
+            gc[H * DIM + d] = phi[B] * g[B * DIM + d] + theta * sh[d];
+        }
+        for (int i = 0; i < NQ; ++i)
+            if (i != B && i != H)
+                for (int d = 0; d < DIM; ++d) gc[i * DIM + d] = theta * phi[i] * g[i * DIM + d];
+    }
+}
+
+// WBPositivityReconstructor::Reconstruct, Reconstruction.hpp:256-287: eta and b extrapolated
+// with the effective gradients, h_f = max(eta_f - b_f, 0), momenta zeroed where h_f < eps_wet
+void reconstruct_wb(const T* q, const T* grad, const double* xc, const double* xf, double eps_wet, T* out) {
+    if (!grad) {
+        for (int i = 0; i < NQ; ++i) out[i] = q[i];
+        return;
+    }
+    T dx[3];
+    for (int d = 0; d < DIM; ++d) dx[d] = xf[d] - xc[d];
+    auto dot = [&](int slot) {
+        T s = (T)0;
+        for (int d = 0; d < DIM; ++d) s += grad[slot * DIM + d] * dx[d];
+        return s;
+    };
+    const T b_bar = q[0], h_bar = q[1];
+    const T b_f = b_bar + dot(0);
+    const T eta_f = (b_bar + h_bar) + dot(1);
+    const T h_f = std::max(eta_f - b_f, (T)0);
+    for (int i = 0; i < NQ; ++i) {
+        if (i == 0)
+            out[i] = b_f;
+        else if (i == 1)
+            out[i] = h_f;
+        else
+            out[i] = (h_f < eps_wet) ? (T)0 : (q[i] + dot(i));
+    }
+}
+
+// the face reconstruction the transport step is configured with (ModularSolver.hpp:46-61)
+void face_state(const T* q, const T* grad, const double* xc, const double* xf, const T* qmin, const T* qmax,
+                int use_limiters, T* out) {
+    if (g_positivity != 0 && grad)
+        reconstruct_wb(q, grad, xc, xf, g_wet_eps, out);
+    else
+        reconstruct(q, grad, xc, xf, qmin, qmax, use_limiters, out);
+}
+
 // TransportStep::ComputeFluxes, TransportStep.hpp:383-565 (owned = all cells on one rank)
 // Extension (well_balanced): after a face's flux, each side adds
 // area * Model::hydrostatic_cell_source(q_face, q_cell, p, outward normal).
@@ -297,12 +424,14 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
         T nh[3] = {0, 0, 0};
         for (int d = 0; d < DIM; ++d) nh[d] = N[d] / area;
         const int cL = m->face_cells[2 * f], cR = m->face_cells[2 * f + 1];
+        // local MOOD: only faces touching a troubled cell (TransportStep.hpp:413-418)
+        if (g_filter && !(*g_filter)[cL] && !(cR >= 0 && (*g_filter)[cR])) continue;
         if (cR >= 0) {
             T qLf[NQ], qRf[NQ], aLf[NA], aRf[NA];
-            reconstruct(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
+            face_state(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
                         m->cell_centroid + 3 * (size_t)cL, xf, vmin ? vmin + (size_t)cL * NQ : nullptr,
                         vmax ? vmax + (size_t)cL * NQ : nullptr, use_limiters, qLf);
-            reconstruct(X + (size_t)cR * NQ, G ? G + (size_t)cR * NQ * DIM : nullptr,
+            face_state(X + (size_t)cR * NQ, G ? G + (size_t)cR * NQ * DIM : nullptr,
                         m->cell_centroid + 3 * (size_t)cR, xf, vmin ? vmin + (size_t)cR * NQ : nullptr,
                         vmax ? vmax + (size_t)cR * NQ : nullptr, use_limiters, qRf);
             for (int i = 0; i < NA; ++i) {
@@ -329,7 +458,7 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
             cell_source(wb, qRf, X + (size_t)cR * NQ, p, nh, -1.0, area, fR);
         } else if (face_bc[f] >= 0) {
             T qLf[NQ], aLf[NA], aRf[NA];
-            reconstruct(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
+            face_state(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
                         m->cell_centroid + 3 * (size_t)cL, xf, vmin ? vmin + (size_t)cL * NQ : nullptr,
                         vmax ? vmax + (size_t)cL * NQ : nullptr, use_limiters, qLf);
             for (int i = 0; i < NA; ++i) aLf[i] = A[(size_t)cL * NA + i];
@@ -346,7 +475,7 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
             cell_source(wb, qLf, X + (size_t)cL * NQ, p, nh, 1.0, area, fL);
         } else if (wb) {
             T qLf[NQ];
-            reconstruct(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
+            face_state(X + (size_t)cL * NQ, G ? G + (size_t)cL * NQ * DIM : nullptr,
                         m->cell_centroid + 3 * (size_t)cL, xf, vmin ? vmin + (size_t)cL * NQ : nullptr,
                         vmax ? vmax + (size_t)cL * NQ : nullptr, use_limiters, qLf);
             cell_source(wb, qLf, X + (size_t)cL * NQ, p, nh, 1.0, area, F + (size_t)cL * NQ);
@@ -382,6 +511,7 @@ void form_rhs(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, 
             gradient_lsq(m, X, w.G.data());
         else
             gradient_gg(m, face_bc, p, X, w.G.data());
+        if (g_positivity != 0) limit_gradients_wb(m, w.Xl.data(), w.G.data(), g_positivity == 1, g_wet_eps);
     }
     compute_fluxes(m, face_bc, p, time, w.Xl.data(), w.Al.data(), grad ? w.G.data() : nullptr,
                    grad ? w.vmin.data() : nullptr, grad ? w.vmax.data() : nullptr, o.use_limiters,
@@ -643,6 +773,25 @@ int orc_form_rhs(const zdm_mesh* m, const int* face_bc, const double* p, int ord
     return 0;
 }
 
+// eta-WB face states (positivity != 0 at order 2): clamp, Green-Gauss gradient of the raw X,
+// LimitGradientsWB, WBPositivityReconstructor per face slot of every cell -> QF[c][k][i]
+// (slots of cell_faces order, max_faces per cell); also the effective gradients G[c][i][d]
+int orc_wb_faces(const zdm_mesh* m, const int* face_bc, const double* p, int skip_theta, double wet_eps,
+                 const double* X, const double* A, double* QF, double* G) {
+    const size_t nc = m->ncells;
+    std::vector<T> Xl(X, X + nc * NQ), Al(A, A + nc * NA);
+    update_state(m->ncells, p, Xl.data(), Al.data());
+    gradient_gg(m, face_bc, p, X, G);
+    limit_gradients_wb(m, Xl.data(), G, skip_theta != 0, wet_eps);
+    for (int c = 0; c < m->ncells; ++c)
+        for (int k = 0; k < m->cell_nfaces[c]; ++k) {
+            const int f = m->cell_faces[(size_t)c * m->max_faces + k];
+            reconstruct_wb(Xl.data() + (size_t)c * NQ, G + (size_t)c * NQ * DIM, m->cell_centroid + 3 * (size_t)c,
+                           m->face_centroid + 3 * (size_t)f, wet_eps, QF + ((size_t)c * m->max_faces + k) * NQ);
+        }
+    return 0;
+}
+
 int orc_flux(const double* qL, const double* qR, const double* aL, const double* aR, const double* p,
              const double* n, double* flux, double* fluct) {
     auto f = NUM::numerical_flux(qL, qR, aL, aR, p, n);
@@ -694,15 +843,39 @@ int orc_ts_step(const zdm_mesh* m, const int* face_bc, const double* p, int orde
     return 0;
 }
 
-// MUSCLSolver::Run main loop, MUSCLSolver.hpp:54, 103, 140-188 (splitting, no MOOD):
+// solver.positivity / wet_dry_eps for the following calls (0 none, 1 mood, 2 zhang_shu); also
+// resets the MOOD override count
+int orc_set_positivity(int positivity, double wet_dry_eps) {
+    g_positivity = positivity;
+    g_wet_eps = wet_dry_eps;
+    g_troubled = 0;
+    g_last_mask.clear();
+    return 0;
+}
+long long orc_troubled_count() { return g_troubled; }
+// the cells the last MOOD override replaced (0 / 1 per cell; all 0 before any override)
+int orc_last_troubled(int nc, int* mask) {
+    for (int c = 0; c < nc; ++c) mask[c] = c < (int)g_last_mask.size() ? g_last_mask[c] : 0;
+    return 0;
+}
+
+// MUSCLSolver::Run main loop, MUSCLSolver.hpp:54, 103, 140-188 (splitting):
 //   constraints; dt = max(CFL, min_dt); while t < t_end and step < max_steps:
-//     clamp dt to t_end; TSStep; constraints; source(dt); constraints; t += dt; dt = CFL
+//     clamp dt to t_end; [save X;] TSStep; constraints; [MOOD;] source(dt); constraints;
+//     t += dt; dt = CFL
+// MOOD (positivity "mood", MUSCLSolver.hpp:130-171, 198-232): cells with h < -1e-10 after the
+// constraints are overwritten by X_save + dt F, F = the first-order (PCM) right-hand side of
+// the saved state over the faces touching them (FormRHSTroubledO1, TransportStep.hpp:363-381),
+// then the constraints again.  zhang_shu at order >= 2 caps the CFL at 1/6 (MUSCLSolver.hpp:31-38).
 // Returns the number of steps taken; *time and *dt_next are updated.
 int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, int ssp_stages,
             int use_limiters, int gradient, int well_balanced, double cfl, double t_end,
             double min_dt, int max_steps, double* X, double* A, double* time, double* dt_io,
             int first) {
+    if (g_positivity == 2 && order >= 2 && cfl > 1.0 / 6.0) cfl = 1.0 / 6.0;
     Opts o{order, ssp_stages, use_limiters, gradient, well_balanced, cfl, t_end, min_dt};
+    Opts o1 = o;
+    o1.order = 1;
     Work w;
     double t = *time;
     double dt = *dt_io;
@@ -710,12 +883,38 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
         enforce(m->ncells, p, X, A);
         dt = std::max(compute_time_step(m, p, X, A, cfl, nullptr), min_dt);
     }
+    const size_t n = (size_t)m->ncells * NQ;
+    std::vector<T> Xs, Fm;
+    std::vector<uint8_t> mask;
     int steps = 0;
     while (t < t_end && steps < max_steps) {
         double t0 = t, dt0 = dt;
         if (t0 + dt0 > t_end) dt0 = t_end - t0;
+        if (g_positivity == 1) Xs.assign(X, X + n);
         ts_step(m, face_bc, p, o, t0, dt0, X, A, w);
         enforce(m->ncells, p, X, A);
+        if (g_positivity == 1) {
+            mask.assign(m->ncells, 0);
+            int nt = 0;
+            for (int c = 0; c < m->ncells; ++c)
+                if (X[(size_t)c * NQ + 1] < -1.0e-10) {
+                    mask[c] = 1;
+                    ++nt;
+                }
+            if (nt > 0) {
+                Fm.assign(n, 0.0);
+                g_filter = &mask;
+                form_rhs(m, face_bc, p, o1, t0, Xs.data(), A, Fm.data(), w);
+                g_filter = nullptr;
+                for (int c = 0; c < m->ncells; ++c)
+                    if (mask[c])
+                        for (int i = 0; i < NQ; ++i)
+                            X[(size_t)c * NQ + i] = Xs[(size_t)c * NQ + i] + dt0 * Fm[(size_t)c * NQ + i];
+                enforce(m->ncells, p, X, A);
+                g_troubled += nt;
+                g_last_mask = mask;
+            }
+        }
         source_solve(m->ncells, p, dt0, X, A);
         enforce(m->ncells, p, X, A);
         ++steps;

@@ -67,6 +67,10 @@ class Oracle:
         L.orc_ts_step.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                   _dp, _dp]
+        L.orc_set_positivity.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_last_troubled.argtypes = [ctypes.c_int, _ip]
+        L.orc_wb_faces.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_troubled_count.restype = ctypes.c_longlong
         L.orc_run.argtypes = [M, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
@@ -106,11 +110,23 @@ class Oracle:
         return out[0]
 
     def form_rhs(self, mesh, face_bc, p, X, A, order=2, use_limiters=1, gradient=0, t=0.0,
-                 well_balanced=0):
+                 well_balanced=0, positivity="none", wet_dry_eps=1e-2):
         F = np.zeros_like(X)
-        self.L.orc_form_rhs(mesh.handle, ip(face_bc), dp(p), order, use_limiters, gradient,
-                            int(well_balanced), t, dp(X), dp(A), dp(F))
+        self.L.orc_set_positivity(self.POSITIVITY[positivity], float(wet_dry_eps))
+        try:
+            self.L.orc_form_rhs(mesh.handle, ip(face_bc), dp(p), order, use_limiters, gradient,
+                                int(well_balanced), t, dp(X), dp(A), dp(F))
+        finally:
+            self.L.orc_set_positivity(0, 1e-2)
         return F
+
+    def wb_faces(self, mesh, face_bc, p, X, A, skip_theta, wet_dry_eps=1e-2):
+        """eta-WB face states [ncells, max_faces, n_q] and effective gradients."""
+        QF = np.zeros((mesh.ncells, mesh.max_faces, self.n_q))
+        G = np.zeros((mesh.ncells, self.n_q, self.dim))
+        self.L.orc_wb_faces(mesh.handle, ip(face_bc), dp(p), int(skip_theta), float(wet_dry_eps),
+                            dp(np.ascontiguousarray(X)), dp(np.ascontiguousarray(A)), dp(QF), dp(G))
+        return QF, G
 
     def gradient(self, mesh, face_bc, p, X, method=0):
         G = np.zeros((mesh.ncells, self.n_q, self.dim))
@@ -132,14 +148,25 @@ class Oracle:
         fin = ctypes.c_int()
         return self.L.orc_compute_dt(mesh.handle, dp(p), dp(X), dp(A), cfl, ctypes.byref(fin))
 
+    POSITIVITY = {"none": 0, "mood": 1, "zhang_shu": 2}
+
     def run(self, mesh, face_bc, p, X, A, order=2, ssp_stages=2, use_limiters=1, gradient=0,
             cfl=0.45, t_end=1e9, min_dt=1e-12, max_steps=10, time=0.0, dt=0.0, first=True,
-            well_balanced=0):
+            well_balanced=0, positivity="none", wet_dry_eps=1e-2):
+        """MUSCLSolver loop; returns (steps, time, next dt).  self.troubled = MOOD overrides."""
         t = np.array([time])
         d = np.array([dt])
-        n = self.L.orc_run(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters,
-                           gradient, int(well_balanced), cfl, t_end, min_dt, max_steps, dp(X),
-                           dp(A), dp(t), dp(d), int(first))
+        self.L.orc_set_positivity(self.POSITIVITY[positivity], float(wet_dry_eps))
+        try:
+            n = self.L.orc_run(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters,
+                               gradient, int(well_balanced), cfl, t_end, min_dt, max_steps, dp(X),
+                               dp(A), dp(t), dp(d), int(first))
+            self.troubled = int(self.L.orc_troubled_count())
+            mask = np.zeros(mesh.ncells, dtype=np.int32)
+            self.L.orc_last_troubled(mesh.ncells, ip(mask))
+            self.last_troubled = mask.astype(bool)
+        finally:
+            self.L.orc_set_positivity(0, 1e-2)
         return n, t[0], d[0]
 
 

@@ -276,7 +276,10 @@ class _Options(ctypes.Structure):
                 ("use_limiters", ctypes.c_int), ("gradient", ctypes.c_int),
                 ("source", ctypes.c_int), ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
                 ("min_dt", ctypes.c_double), ("block_size", ctypes.c_int),
-                ("well_balanced", ctypes.c_int)]
+                ("well_balanced", ctypes.c_int), ("positivity", ctypes.c_int),
+                ("wet_dry_eps", ctypes.c_double)]
+
+POSITIVITY = {"none": 0, "mood": 1, "zhang_shu": 2}
 
 
 _model_libs = {}
@@ -292,7 +295,7 @@ SOLVER_SYMBOLS = [
     "zdm_chorin_setup", "zdm_chorin_step", "zdm_chorin_pressure", "zdm_vam_frozen_aux",
     "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres", "zdm_batch_create", "zdm_batch_time",
     "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
-    "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout",
+    "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout", "zdm_troubled_cells",
 ]
 
 
@@ -348,6 +351,7 @@ def model_lib(model):
     lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_residuals.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_layout.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.zdm_troubled_cells.argtypes = [V, ctypes.POINTER(ctypes.c_longlong)]
     K = ctypes.POINTER(KrylovOptions)
     lib.zdm_chorin_setup.argtypes = [V, K]
     lib.zdm_chorin_step.argtypes = [V, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
@@ -394,7 +398,8 @@ class Solver:
 
     def __init__(self, model, mesh, settings=None, params=None, face_bc=None, source=True,
                  order=None, ssp_stages=None, cfl=None, t_end=None, gradient=None,
-                 use_limiters=None, well_balanced=None, cell_offsets=None):
+                 use_limiters=None, well_balanced=None, cell_offsets=None, positivity=None,
+                 wet_dry_eps=None):
         self.model = model
         self.info = ModelInfo(model)
         self.lib = model_lib(model)
@@ -433,6 +438,11 @@ class Solver:
         o.block_size = 0
         o.well_balanced = int(bool(well_balanced if well_balanced is not None
                                    else sv.get("well_balanced", False)))
+        pos = positivity if positivity is not None else sv.get("positivity", "none")
+        if pos not in POSITIVITY:
+            raise ZdmError("positivity must be one of %s" % sorted(POSITIVITY))
+        o.positivity = POSITIVITY[pos]
+        o.wet_dry_eps = float(wet_dry_eps if wet_dry_eps is not None else sv.get("wet_dry_eps", 1e-2))
         self.options = o
         h = ctypes.c_void_p()
         if cell_offsets is None:
@@ -550,6 +560,12 @@ class Solver:
         self._check(self.lib.zdm_adjoint_setup(self._h, int(nsteps), int(checkpoint_every),
                                                int(obs_every), len(g), g.ctypes.data_as(_c_int_p),
                                                _dp(o) if o.size else None, int(bool(exact_limiter))))
+
+    def troubled_cells(self):
+        """MOOD overrides since the last initialize (summed over ensemble members)."""
+        n = ctypes.c_longlong(0)
+        self._check(self.lib.zdm_troubled_cells(self._h, ctypes.byref(n)))
+        return n.value
 
     def adjoint_layout(self):
         """(checkpoint_every, tape level) chosen by adjoint_setup."""

@@ -138,7 +138,7 @@ struct Ctrl {
     int active;   // t < t_end
     int steps;
     int finite;
-    int pad;
+    int troubled; // MOOD overrides so far
 };
 
 struct DevMesh {
@@ -223,30 +223,31 @@ __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
 // GRAD: 0 = none (PCM), 1 = Green-Gauss, 2 = least squares
 // 3 waves / SIMD: the hoisted neighbour loads need ~170 VGPRs; measured faster than the
 // 4-wave form with the loads issued inside the face loop.
-template <int MAXF, int GRAD>
-__global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
-                                                const T* __restrict__ X, const T* __restrict__ Ag,
-                                                T* __restrict__ As, T* __restrict__ QF, int use_limiters) {
-    const Loc L = locate(m);
-    ctrl += L.m;
-    if (!ctrl->active) return;
-    const int c = L.c;
-    if (c >= m.ncells) return;
+// Clamp + aux of cell c (UpdateState), its gradient of the raw state (GRAD 1: Green-Gauss,
+// 2: least squares, 3: Green-Gauss with the eta-WB limiting of positivity "mood" / "zhang_shu"),
+// the neighbour bounds of the clamped states and the limited linear reconstruction at every face
+// of the cell (Reconstruction.hpp:203-243; GRAD 3: WBPositivityReconstructor, 256-287).
+// put(k, i, v) receives component i of face slot k's state; the clamped aux is returned in ac.
+// wet_eps / zs_theta: GRAD 3 only (wet_dry_eps; 1 = apply the Zhang-Shu theta).
+template <int MAXF, int GRAD, typename Put>
+__device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, const T* __restrict__ X,
+                                           const T* __restrict__ Ag, int c, int use_limiters, T* ac, Put put,
+                                           T wet_eps = 0, int zs_theta = 0) {
+    constexpr bool GG = GRAD == 1 || GRAD == 3;
     const int ld = m.ld;
-    T q[NQ], qc[NQ], ac[NA];
+    T q[NQ], qc[NQ];
     load_soa<NQ>(X, ld, c, q);
     load_soa<NA>(Ag, ld, c, ac);
 #pragma unroll
     for (int i = 0; i < NQ; ++i) qc[i] = q[i];
     update_state_cell(P, qc, ac);
-    store_soa<NA>(As, ld, c, ac);
 
     if constexpr (GRAD == 0) {
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             if (m.cface[(size_t)k * ld + c] < 0) continue;
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) QF[((size_t)k * NQ + i) * ld + c] = qc[i];
+            for (int i = 0; i < NQ; ++i) put(k, i, qc[i]);
         }
         return;
     } else {
@@ -274,8 +275,10 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
         T lo[NQ], hi[NQ], g[NQ * DIM];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
-            lo[i] = qc[i];
-            hi[i] = qc[i];
+            // GRAD 3: extrema of W = (b, eta = b + h, momenta), LimitGradientsWB
+            const T w = (GRAD == 3 && i == 1) ? qc[0] + qc[1] : qc[i];
+            lo[i] = w;
+            hi[i] = w;
         }
 #pragma unroll
         for (int i = 0; i < NQ * DIM; ++i) g[i] = 0;
@@ -291,10 +294,11 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
                 auto rn = M::update_variables(qn[k], an[k], P.p, 0.0);
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
-                    if (rn[i] < lo[i]) lo[i] = rn[i];
-                    if (rn[i] > hi[i]) hi[i] = rn[i];
+                    const T w = (GRAD == 3 && i == 1) ? rn[0] + rn[1] : rn[i];
+                    if (w < lo[i]) lo[i] = w;
+                    if (w > hi[i]) hi[i] = w;
                 }
-                if constexpr (GRAD == 1) {
+                if constexpr (GG) {
                     const bool right = (mk[k] >> 2) & 1;
 #pragma unroll
                     for (int i = 0; i < NQ; ++i) {
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
                             g[i * DIM + d] += m.lsqw[((size_t)k * DIM + d) * ld + c] * du;
                     }
                 }
-            } else if constexpr (GRAD == 1) {
+            } else if constexpr (GG) {
                 const int bc = m.fbc[f];
                 if (bc >= 0) {
                     const T nh[3] = {m.fnhx[f], DIM > 1 ? m.fnhy[f] : (T)0, 0};
@@ -330,12 +334,103 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
                 }
             }
         }
-        if constexpr (GRAD == 1) {
+        if constexpr (GG) {
             const T vol = m.vol[c];
             if (vol > 1e-15) {
 #pragma unroll
                 for (int i = 0; i < NQ * DIM; ++i) g[i] /= vol;
             }
+        }
+        if constexpr (GRAD == 3) {
+            // TransportStep::LimitGradientsWB (TransportStep.hpp:241-323) on the raw gradient g:
+            // Venkatakrishnan phi per W variable over the cell's faces (eps^2 = vol^(2/dim)),
+            // first order below wet_eps, b and momentum slopes tied to eta, Zhang-Shu theta on
+            // sh = phi_eta grad eta - phi_b grad b; then WBPositivityReconstructor
+            // (Reconstruction.hpp:256-287) at every face
+            constexpr int B = 0, H = 1;
+            const T vol = m.vol[c];
+            const T eps2 = DIM == 2 ? vol : vol * vol;
+            T geta[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) geta[d] = g[H * DIM + d] + g[B * DIM + d];
+            const T h_bar = qc[H], b_bar = qc[B];
+            T phi[NQ];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) phi[i] = 1.0;
+#pragma unroll
+            for (int k = 0; k < MAXF; ++k) {
+                if (fk[k] < 0) continue;
+                const T r[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    T delta = 0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) delta += ((i == H) ? geta[d] : g[i * DIM + d]) * r[d];
+                    if (std::fabs(delta) < 1e-14) continue;
+                    const T wc = (i == H) ? b_bar + h_bar : qc[i];
+                    const T dm = (delta > 0) ? (hi[i] - wc) : (lo[i] - wc);
+                    const T num = dm * dm + eps2 + 2 * delta * dm, den = dm * dm + 2 * delta * delta + delta * dm + eps2;
+                    T pf = (den > 1e-30) ? num / den : (T)1.0;
+                    pf = std::max((T)0, std::min((T)1, pf));
+                    if (pf < phi[i]) phi[i] = pf;
+                }
+            }
+            if (h_bar < wet_eps) {
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) phi[i] = 0.0;
+            }
+            phi[B] = std::min(phi[B], phi[H]);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                if (i != B && i != H) phi[i] = std::min(phi[i], phi[H]);
+            T sh[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) sh[d] = phi[H] * geta[d] - phi[B] * g[B * DIM + d];
+            T hmin = h_bar;
+#pragma unroll
+            for (int k = 0; k < MAXF; ++k) {
+                if (fk[k] < 0) continue;
+                const T r[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
+                T hf = h_bar;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) hf += sh[d] * r[d];
+                if (hf < hmin) hmin = hf;
+            }
+            T theta = 1.0;
+            if (zs_theta && hmin < 0.0) theta = std::max((T)0, std::min((T)1, h_bar / std::max(h_bar - hmin, (T)1e-14)));
+            // effective W-gradients
+            T ge[NQ * DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                ge[B * DIM + d] = phi[B] * g[B * DIM + d];
+                ge[H * DIM + d] = phi[B] * g[B * DIM + d] + theta * sh[d];
+            }
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                if (i != B && i != H)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) ge[i * DIM + d] = theta * phi[i] * g[i * DIM + d];
+#pragma unroll
+            for (int k = 0; k < MAXF; ++k) {
+                if (fk[k] < 0) continue;
+                const T dx[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
+                T dot[NQ];
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    T sd = 0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) sd += ge[i * DIM + d] * dx[d];
+                    dot[i] = sd;
+                }
+                const T b_f = b_bar + dot[B];
+                const T eta_f = (b_bar + h_bar) + dot[H];
+                const T h_f = std::max(eta_f - b_f, (T)0);
+                put(k, B, b_f);
+                put(k, H, h_f);
+#pragma unroll
+                for (int i = 2; i < NQ; ++i) put(k, i, (h_f < wet_eps) ? (T)0 : (qc[i] + dot[i]));
+            }
+            return;
         }
         // limited linear reconstruction at every face of the cell (Reconstruction.hpp:203-243)
 #pragma unroll
@@ -382,42 +477,43 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
             for (int i = 0; i < NQ; ++i)
                 if (alpha > fin[i]) alpha = fin[i];
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) QF[((size_t)k * NQ + i) * ld + c] = qc[i] + alpha * delta[i];
+            for (int i = 0; i < NQ; ++i) put(k, i, qc[i] + alpha * delta[i]);
         }
     }
 }
 
-// ------------------------------------------------------------------------------ k_flux
-enum { MODE_RHS = 0, MODE_EULER = 1, MODE_STAGE = 2, MODE_FINAL = 3 };
-
-template <int MAXF, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
-                                                int stage, const T* __restrict__ QF,
-                                                const T* __restrict__ As, const T* In, T* Out,
-                                                const T* __restrict__ Ag, int wb, const T* X0) {
+template <int MAXF, int GRAD>
+__global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
+                                                const T* __restrict__ X, const T* __restrict__ Ag,
+                                                T* __restrict__ As, T* __restrict__ QF, int use_limiters,
+                                                T wet_eps, int zs_theta) {
     const Loc L = locate(m);
     ctrl += L.m;
     if (!ctrl->active) return;
     const int c = L.c;
     if (c >= m.ncells) return;
     const int ld = m.ld;
-    const T time = ctrl->stage_t[stage];
-    T fc[NQ], ac[NA];
+    T ac[NA];
+    recon_cell<MAXF, GRAD>(
+        m, P, X, Ag, c, use_limiters, ac, [&](int k, int i, T v) { QF[((size_t)k * NQ + i) * ld + c] = v; },
+        wet_eps, zs_theta);
+    store_soa<NA>(As, ld, c, ac);
+}
+
+// ------------------------------------------------------------------------------ k_flux
+enum { MODE_RHS = 0, MODE_EULER = 1, MODE_STAGE = 2, MODE_FINAL = 3 };
+
+// The face loop of ComputeFluxes for cell c (TransportStep.hpp:383-565): per face slot in
+// ascending order the face aux, flux and fluctuations of (own, neighbour) face states and the
+// -D A -/+ F A accumulation, boundary ghosts, the optional well-balanced cell term; then / vol.
+// own(k, qo) loads face slot k's own state, nbr(k, nb, ks, qn, an) the neighbour's state and
+// clamped aux; ac = the cell's clamped aux, qcell = its clamped state (well-balanced term only).
+template <int MAXF, typename Own, typename Nbr>
+__device__ __forceinline__ void flux_cell(const DevMesh& m, const Params& P, int c, T time, const T* ac, int wb,
+                                          const T* qcell, Own own, Nbr nbr, T* fc) {
+    const int ld = m.ld;
 #pragma unroll
     for (int i = 0; i < NQ; ++i) fc[i] = 0;
-    load_soa<NA>(As, ld, c, ac);
-    T qcell[NQ];   // clamped cell state (only read by the well-balanced term)
-    if constexpr (HasCellSource<T>::value) {
-        if (wb) {
-            T a0[NA];
-            load_soa<NQ>(In, ld, c, qcell);
-            load_soa<NA>(Ag, ld, c, a0);
-            auto rq = M::update_variables(qcell, a0, P.p, 0.0);
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
-        }
-    }
-
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) {
         const int f = m.cface[(size_t)k * ld + c];
@@ -428,16 +524,13 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
         const T nh[3] = {m.fnhx[f], DIM > 1 ? m.fnhy[f] : (T)0, 0};
         const T xf[3] = {m.fcx[f], DIM > 1 ? m.fcy[f] : (T)0, 0};
         T qo[NQ];
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) qo[i] = QF[((size_t)k * NQ + i) * ld + c];
+        own(k, qo);
         if (nb >= 0) {
             const int meta = m.cmeta[(size_t)k * ld + c];
             const int ks = meta & 3;
             const bool right = (meta >> 2) & 1;
             T qn[NQ], an[NA];
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
-            load_soa<NA>(As, ld, nb, an);
+            nbr(k, nb, ks, qn, an);
             const T* qL = right ? qn : qo;
             const T* qR = right ? qo : qn;
             const T* aLc = right ? an : ac;
@@ -489,6 +582,12 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
 #pragma unroll
         for (int i = 0; i < NQ; ++i) fc[i] /= vol;
     }
+}
+
+// Out = the stage update of the cell from its right-hand side fc (MODE_RHS: fc itself)
+template <int MODE>
+__device__ __forceinline__ void stage_update(const Ctrl* ctrl, int c, int ld, const T* fc, const T* In, T* Out,
+                                             const T* X0) {
     if constexpr (MODE == MODE_RHS) {
         store_soa<NQ>(Out, ld, c, fc);
     } else if constexpr (MODE == MODE_EULER || MODE == MODE_STAGE) {
@@ -503,6 +602,46 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
         for (int i = 0; i < NQ; ++i)
             Out[(size_t)i * ld + c] = al * In[(size_t)i * ld + c] + be * fc[i] + ga * X0[(size_t)i * ld + c];
     }
+}
+
+template <int MAXF, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
+                                                int stage, const T* __restrict__ QF,
+                                                const T* __restrict__ As, const T* In, T* Out,
+                                                const T* __restrict__ Ag, int wb, const T* X0) {
+    const Loc L = locate(m);
+    ctrl += L.m;
+    if (!ctrl->active) return;
+    const int c = L.c;
+    if (c >= m.ncells) return;
+    const int ld = m.ld;
+    const T time = ctrl->stage_t[stage];
+    T fc[NQ], ac[NA];
+    load_soa<NA>(As, ld, c, ac);
+    T qcell[NQ];   // clamped cell state (only read by the well-balanced term)
+    if constexpr (HasCellSource<T>::value) {
+        if (wb) {
+            T a0[NA];
+            load_soa<NQ>(In, ld, c, qcell);
+            load_soa<NA>(Ag, ld, c, a0);
+            auto rq = M::update_variables(qcell, a0, P.p, 0.0);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
+        }
+    }
+    flux_cell<MAXF>(
+        m, P, c, time, ac, wb, qcell,
+        [&](int k, T* qo) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qo[i] = QF[((size_t)k * NQ + i) * ld + c];
+        },
+        [&](int k, int nb, int ks, T* qn, T* an) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
+            load_soa<NA>(As, ld, nb, an);
+        },
+        fc);
+    stage_update<MODE>(ctrl, c, ld, fc, In, Out, X0);
 }
 
 // ------------------------------------------------------------------------------ source
@@ -679,6 +818,65 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
     }
 }
 
+// Post-stage of a step with the local MOOD override (MUSCLSolver.hpp:150-182): constraints on the
+// transport result X; a cell left with h < -1e-10 is replaced by its first-order forward-Euler
+// update from the saved start-of-step state Xs, X_save + dt F (F = k_flux's face loop over the
+// clamped saved states of the cell and its neighbours, FormRHSTroubledO1, TransportStep.hpp:
+// 363-381, at the step's start time), then constraints again; then [source, constraints,] CFL as
+// k_post.  Cells are independent: Xs and As are read-only here, X and A are written per cell.
+// Untroubled cells get one more clamp in the reference, a no-op (the clamp is idempotent).
+template <int MAXF, int SRC>
+__global__ __launch_bounds__(BLOCK) void k_post_mood(DevMesh m, Params P, Ctrl* __restrict__ ctrl, T* X,
+                                                     T* A, const T* __restrict__ Xs,
+                                                     const T* __restrict__ As, T* __restrict__ partials) {
+    const Loc L = locate(m);
+    ctrl += L.m;
+    if (!ctrl->active) return;
+    const int c = L.c;
+    const int ld = m.ld;
+    T ratio = DBL_MAX;
+    if (c < m.ncells) {
+        T q[NQ], a[NA];
+        load_soa<NQ>(X, ld, c, q);
+        load_soa<NA>(A, ld, c, a);
+        update_state_cell(P, q, a);
+        if (q[1] < (T)-1.0e-10) {
+            T qs[NQ], as[NA], qsc[NQ], fc[NQ];
+            load_soa<NQ>(Xs, ld, c, qs);
+            load_soa<NA>(As, ld, c, as);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qsc[i] = qs[i];
+            update_state_cell(P, qsc, as);
+            flux_cell<MAXF>(
+                m, P, c, ctrl->t, as, 0, nullptr,
+                [&](int k, T* qo) {
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i) qo[i] = qsc[i];
+                },
+                [&](int k, int nb, int ks, T* qn, T* an) {
+                    load_soa<NQ>(Xs, ld, nb, qn);
+                    load_soa<NA>(As, ld, nb, an);
+                    update_state_cell(P, qn, an);
+                },
+                fc);
+            const T dt = ctrl->dt;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) q[i] = qs[i] + dt * fc[i];
+            update_state_cell(P, q, a);
+            atomicAdd(&ctrl->troubled, 1);
+        }
+        if constexpr (SRC) {
+            newton_cell(P, ctrl->dt, q, a);
+            update_state_cell(P, q, a);
+        }
+        store_soa<NQ>(X, ld, c, q);
+        store_soa<NA>(A, ld, c, a);
+        ratio = cfl_ratio(P, q, a, m.rc[c]);
+    }
+    ratio = block_min(ratio);
+    if (threadIdx.x == 0) partials[L.lb] = ratio;
+}
+
 // k_post<POST> (POST 1: constraints; POST 2: constraints, source, constraints) of a step whose
 // intermediate states are kept: reads the final stage's output X1, writes the source's solution
 // X3 and the aux A3 it saw (POST 2), the end-of-step state Xout and its aux A, and the CFL
@@ -815,6 +1013,8 @@ struct zdm_solver {
     size_t arena_bytes;
     DevMesh dm;
     T *X, *W, *F, *A, *As, *Atmp, *QF, *partials, *stage_aos;
+    T* Xs;                            // MOOD: the state at the start of the step
+    double cfl_in;                    // solver.cfl as given (opt.cfl: capped for zhang_shu)
     Ctrl *ctrl, *ctrl_rhs;
     bool timing;
     std::vector<hipEvent_t> evpool;   // recorded in order; pending until resolved
@@ -865,13 +1065,16 @@ bool chol_solve_host(int n, std::vector<T>& A, std::vector<T>& b) {
 template <int MAXF>
 void launch_prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* As, T* QF) {
     dim3 g(s->nblocks), b(BLOCK);
-    const int lim = s->opt.use_limiters;
+    const int lim = s->opt.use_limiters, pos = s->opt.positivity;
+    const T eps = s->opt.wet_dry_eps;
     if (s->opt.order < 2)
-        k_prep<MAXF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim);
+        k_prep<MAXF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+    else if (pos != 0)
+        k_prep<MAXF, 3><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, pos == 2);
     else if (s->opt.gradient == 1)
-        k_prep<MAXF, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim);
+        k_prep<MAXF, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
     else
-        k_prep<MAXF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim);
+        k_prep<MAXF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
 }
 
 // MODE_FINAL: Out = al In + be F(In) + ga X0 (X0 = Out in place)
@@ -919,6 +1122,25 @@ void flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* 
         launch_flux<3>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
     else
         launch_flux<4>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
+}
+
+// post-stage with the MOOD override (positivity "mood"): reads the saved state s->Xs
+void post_mood(zdm_solver* s) {
+    dim3 g(s->nblocks), b(BLOCK);
+    const bool src = s->opt.source;
+#define ZDM_MOOD(MF)                                                                                                 \
+    if (src)                                                                                                         \
+        k_post_mood<MF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->Xs, s->As, s->partials);   \
+    else                                                                                                             \
+        k_post_mood<MF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->Xs, s->As, s->partials);
+    if (s->maxf <= 2) {
+        ZDM_MOOD(2)
+    } else if (s->maxf == 3) {
+        ZDM_MOOD(3)
+    } else {
+        ZDM_MOOD(4)
+    }
+#undef ZDM_MOOD
 }
 
 void post(zdm_solver* s, int mode, T dt_override = 0) {
@@ -1028,12 +1250,22 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     if (opts->block_size != 0 && opts->block_size != BLOCK) return fail("block_size must be 256");
     if (opts->well_balanced && !HasCellSource<T>::value)
         return fail("well_balanced needs Model::hydrostatic_cell_source");
+    if (opts->positivity < 0 || opts->positivity > 2) return fail("positivity must be 0 (none), 1 (mood) or 2 (zhang_shu)");
+    if (opts->positivity != 0 && opts->order >= 2 && opts->gradient != 0)
+        return fail("positivity needs the Green-Gauss gradient (ModularSolver.hpp:62)");
+    if (opts->positivity != 0 && opts->well_balanced)
+        return fail("positivity and the well_balanced extension cannot be combined");
+    if (opts->positivity != 0 && NQ < 2) return fail("positivity needs the [b, h, ...] state layout");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
 
     zdm_solver* s = new zdm_solver();
     std::memset(&s->P, 0, sizeof(Params));
     s->opt = *opts;
+    s->cfl_in = opts->cfl;
+    if (s->opt.wet_dry_eps <= 0) s->opt.wet_dry_eps = 1.0e-2;
+    // Zhang-Shu positivity holds for CFL <= 1/6 (MUSCLSolver.hpp:31-38)
+    if (s->opt.positivity == 2 && s->opt.order >= 2 && s->opt.cfl > 1.0 / 6.0) s->opt.cfl = 1.0 / 6.0;
     auto dp = M::default_parameters();
     for (int i = 0; i < NPAR; ++i) s->P.p[i] = params ? params[i] : dp[i];
     const int nc = mesh->ncells, nf = mesh->nfaces, MF = mesh->max_faces;
@@ -1148,6 +1380,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     const size_t sQ = sizeof(T) * (size_t)NQ * ld, sA = sizeof(T) * (size_t)NA * ld;
     size_t oX = take(sQ), oW = take(sQ), oF = take(sQ), oA = take(sA), oAs = take(sA), oAt = take(sA);
     size_t oQF = take(sizeof(T) * (size_t)MFa * NQ * ld);
+    size_t oXs = take(s->opt.positivity == 1 ? sQ : 8);   // MOOD: the saved start-of-step state
     size_t oPart = take(sizeof(T) * (size_t)s->nblocks);
     size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
     size_t oCtrl = take(sizeof(Ctrl) * s->nctrl), oCtrlR = take(sizeof(Ctrl) * s->nctrl);
@@ -1173,6 +1406,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     }
     char* a = s->arena;
     s->X = (T*)(a + oX);
+    s->Xs = (T*)(a + oXs);
     s->W = (T*)(a + oW);
     s->F = (T*)(a + oF);
     s->A = (T*)(a + oA);
@@ -1398,6 +1632,8 @@ static int resolve_events(zdm_solver* s) {
 // the launches of one MUSCLSolver time step
 static void enqueue_step(zdm_solver* s) {
     const int order = s->opt.order, st = s->opt.ssp_stages;
+    const bool mood = s->opt.positivity == 1;
+    if (mood) (void)hipMemcpyAsync(s->Xs, s->X, sizeof(T) * (size_t)NQ * s->ld, hipMemcpyDeviceToDevice, s->stream);
     record(s, -1);
     if (order < 2) {
         prep(s, s->ctrl, s->X, s->A);
@@ -1417,7 +1653,10 @@ static void enqueue_step(zdm_solver* s) {
         flux(s, s->ctrl, MODE_FINAL, st - 1, s->W, s->X, s->A);
         record(s, 1);
     }
-    post(s, s->opt.source ? 2 : 1);
+    if (mood)
+        post_mood(s);
+    else
+        post(s, s->opt.source ? 2 : 1);
     record(s, 2);
     finalize(s, 2);
 }
@@ -1687,6 +1926,9 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
 
 int zdm_chorin_setup(zdm_solver* s, const zdm_krylov_options* opts) {
     if (!s || !s->ch) return fail("null solver");
+    // ChorinVAMSolver::Run (ChorinVAM.hpp:86-129) has neither MOOD nor the zhang_shu CFL cap of
+    // MUSCLSolver::Run; its predictor keeps the eta-WB reconstruction at order 2
+    s->opt.cfl = s->cfl_in;
     const zdm_krylov_options o = opts ? *opts : krylov_defaults();
     if (check_krylov(o)) return -1;
     ChorinHost* ch = s->ch;
@@ -1804,6 +2046,16 @@ int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_e
     if (!s) return fail("null solver");
     if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs)) return -1;
     s->adj->freeze_limiter = exact_limiter ? 0 : 1;
+    return 0;
+}
+
+int zdm_troubled_cells(zdm_solver* s, long long* count) {
+    if (!s || !count) return fail("zdm_troubled_cells: null argument");
+    std::vector<Ctrl> cs;
+    if (read_ctrls(s, cs)) return -1;
+    long long n = 0;
+    for (const Ctrl& c : cs) n += c.troubled;
+    *count = n;
     return 0;
 }
 
