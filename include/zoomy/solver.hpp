@@ -31,6 +31,7 @@
 
 #include "zdm.h"
 #include "zdm_mesh.h"
+#include "zoomy/io.hpp"
 #include "zoomy/settings.hpp"
 
 namespace zoomy {
@@ -437,6 +438,7 @@ public:
     std::vector<int> face_bc;
     std::vector<double> X, A;   // solver cell order
     bool verbose = true;
+    bool write_snapshots = true;   // io.snapshots VTK output (the CLI's -no_output turns it off)
     int max_steps = 1 << 30;
     double time = 0;
     int steps = 0;
@@ -536,9 +538,24 @@ public:
         ZOOMY_TRY(InitializeComponents());
         ZOOMY_CALL(zdm_state_upload(h_, X.data(), A.data()));
         ZOOMY_CALL(zdm_initialize(h_));
-        int done = 0;
-        ZOOMY_CALL(zdm_run(h_, max_steps, 16, &done));
+        IOManager io(settings);
         double dt = 0;
+        if (write_snapshots && io.enabled()) {
+            // the reference's Monitor after every step (MUSCLSolver.hpp:122, 187)
+            io.PrepareDirectory();
+            ZOOMY_TRY(Monitor(io, 0.0));
+            while (steps < max_steps) {
+                ZOOMY_CALL(zdm_step(h_, 1));
+                int n = 0;
+                ZOOMY_CALL(zdm_get_time(h_, &time, &dt, &n));
+                if (n == steps) break;   // t_end reached
+                steps = n;
+                ZOOMY_TRY(Monitor(io, time));
+            }
+        } else {
+            int done = 0;
+            ZOOMY_CALL(zdm_run(h_, max_steps, 16, &done));
+        }
         ZOOMY_CALL(zdm_get_time(h_, &time, &dt, &steps));
         if (settings.solver.positivity == "mood") {
             long long nt = 0;
@@ -550,6 +567,17 @@ public:
     }
     // MUSCLSolver::EnforcePhysicalConstraints (MUSCLSolver.hpp:234-254)
     ErrorCode EnforcePhysicalConstraints() { return transport->UpdateState(); }
+
+protected:
+    // VirtualSolver::Monitor (VirtualSolver.hpp:181-192): a VTK snapshot when one is due
+    ErrorCode Monitor(IOManager& io, double t) {
+        if (!io.ShouldWrite(t)) return 0;
+        ZOOMY_CALL(zdm_state_download(h_, X.data(), A.data()));
+        std::string err;
+        if (!io.WriteVTK(mesh.get(), X, A, info.n_q, info.n_aux, t, &err)) return Fail(err);
+        io.AdvanceSnapshot();
+        return 0;
+    }
 };
 
 class ChorinVAMSolver : public MUSCLSolver {
@@ -578,8 +606,24 @@ public:
         ZOOMY_CALL(zdm_state_upload(h_, X.data(), A.data()));
         ZOOMY_CALL(zdm_initialize(h_));
         int n = 0, its = 0, reason = 0;
-        ZOOMY_CALL(zdm_chorin_step(h_, max_steps, &n, &its, &reason));
         double dt = 0;
+        IOManager io(settings);
+        if (write_snapshots && io.enabled()) {
+            // WriteSnapshot before the loop and after every step (ChorinVAM.hpp:101, 124)
+            io.PrepareDirectory();
+            ZOOMY_TRY(Monitor(io, 0.0));
+            while (steps < max_steps) {
+                int n1 = 0, its1 = 0, r1 = 0;
+                ZOOMY_CALL(zdm_chorin_step(h_, 1, &n1, &its1, &r1));
+                if (n1 == 0) break;   // t_end reached
+                its += its1;
+                if (r1 < 0 || reason >= 0) reason = r1;
+                ZOOMY_CALL(zdm_get_time(h_, &time, &dt, &steps));
+                ZOOMY_TRY(Monitor(io, time));
+            }
+        } else {
+            ZOOMY_CALL(zdm_chorin_step(h_, max_steps, &n, &its, &reason));
+        }
         ZOOMY_CALL(zdm_get_time(h_, &time, &dt, &steps));
         last_reason = reason;
         total_its = its;

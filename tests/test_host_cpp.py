@@ -80,3 +80,37 @@ def test_cli_errors(tmp_path):
     f.write_text('{"solver": {"t_end": "late"}}')
     r = subprocess.run([CLI, "-settings", str(f), "-print-settings"], capture_output=True, text=True)
     assert r.returncode == 1 and "number" in r.stderr
+
+
+def test_vtu_snapshots_written_by_the_io_manager(tmp_path):
+    """zoomy::IOManager (IOManager.hpp:19-120, 346-376) on the CPU: snapshots every t_end /
+    snapshots from time 0, '<filename>-NNN.vtu' with the mesh in its file's cell order and the
+    cell data, and the .vtu.series listing."""
+    import shutil
+    import numpy as np
+    from test_gpu_host_cpp import MESH, read_vtu
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path / "vtu_write")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "vtu_write.cpp"), "-L", PKG_DIR, "-lzdm_mesh",
+                    "-Wl,-rpath," + PKG_DIR, "-o", exe], check=True, capture_output=True, text=True)
+    r = subprocess.run([exe, MESH, str(tmp_path / "o")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    series = json.loads((tmp_path / "o" / "snap.vtu.series").read_text())
+    assert [f["name"] for f in series["files"]] == ["snap-000.vtu", "snap-001.vtu", "snap-002.vtu"]
+    assert [f["time"] for f in series["files"]] == [0.0, 0.5, 1.0]
+    v = read_vtu(tmp_path / "o" / "snap-001.vtu")
+    mesh = zdm.Mesh.read_msh(MESH)
+    n = mesh.ncells
+    assert np.array_equal(v["State"], 10.0 * np.arange(n)[:, None] + np.arange(3)[None, :])
+    assert np.array_equal(v["Aux"], -np.arange(n, dtype=np.float64))
+    nvc = np.diff(np.concatenate([[0], v["offsets"]]))
+    assert len(nvc) == n and np.all(v["types"] == np.where(nvc == 3, 5, 9))
+    # every cell's centroid (file order) lies in the bounding box of its vertices
+    P = v["Points"].reshape(-1, 3)
+    cen = np.empty((n, 3))
+    cen[mesh.cell_perm] = mesh.cell_centroid
+    for i in range(n):
+        vs = P[v["connectivity"][v["offsets"][i] - nvc[i]:v["offsets"][i]]]
+        assert np.all(vs.min(axis=0)[:2] - 1e-12 <= cen[i, :2]) and np.all(cen[i, :2] <= vs.max(axis=0)[:2] + 1e-12)

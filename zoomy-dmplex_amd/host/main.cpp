@@ -2,6 +2,7 @@
 // library: reads -settings (default settings.json), picks ChorinVAMSolver for
 // solver.method == "chorin" and MUSCLSolver otherwise, and runs to t_end.
 //   -o <file.npy>      final state, float64 (ncells, n_q), mesh-file cell order
+//   -no_output         no VTK snapshots (otherwise io.snapshots of them, as the reference)
 //   -max_steps <n>     stop after n steps
 //   -print-settings    print the parsed settings as JSON and exit
 //   -ksp_* / -pc_type  pressure-solve options (Chorin), as the reference reads them
@@ -52,13 +53,14 @@ static void print_settings(const zoomy::Settings& s) {
 int main(int argc, char** argv) {
     std::string spath = "settings.json", out;
     int max_steps = 1 << 30;
-    bool print = false;
+    bool print = false, snapshots = true;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "-settings" && i + 1 < argc) spath = argv[i + 1];
         if (a == "-o" && i + 1 < argc) out = argv[i + 1];
         if (a == "-max_steps" && i + 1 < argc) max_steps = std::stoi(argv[i + 1]);
         if (a == "-print-settings") print = true;
+        if (a == "-no_output") snapshots = false;
     }
     zoomy::Settings st;
     try {
@@ -77,6 +79,7 @@ int main(int argc, char** argv) {
     else
         solver = std::make_unique<zoomy::MUSCLSolver>();
     solver->max_steps = max_steps;
+    solver->write_snapshots = snapshots;
     if (solver->Run(argc, argv)) {
         std::cerr << "Error: " << zoomy::LastError() << std::endl;
         return 2;
