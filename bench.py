@@ -222,6 +222,7 @@ def run_c4(zdm, cases, args):
     ncells = e.mesh.ncells
     e.upload(e.join(states))
     e.initialize()
+    bs, cpt, tune = e.autotune(3)          # launch shape of the batched step, by measurement
     e.step_graph(args.warmup)
     e.synchronize()
     t0 = time.perf_counter()
@@ -233,8 +234,10 @@ def run_c4(zdm, cases, args):
         raise RuntimeError("non-finite ensemble state")
     dts = [e.member_time(i)[1] for i in range(count)]
     e.close()
-    # the same runs one at a time: replaying a captured step, and with plain launches
-    t_serial = t_eager = 0.0
+    # the same runs one at a time: replaying a captured step (default launch shape, and each
+    # member autotuned), and with plain launches
+    t_serial = t_eager = t_tuned = 0.0
+    shapes = {}
     for m, X in zip(meshes, states):
         s = zdm.Solver("sme1", m, **kw)
         s.upload(X)
@@ -245,6 +248,17 @@ def run_c4(zdm, cases, args):
         s.step_graph(args.steps)
         s.synchronize()
         t_serial += time.perf_counter() - t0
+        s.upload(X)
+        s.initialize()
+        mb, mc, _ = s.autotune(3)
+        shapes[(mb, mc)] = shapes.get((mb, mc), 0) + 1
+        s.step_graph(args.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step_graph(args.steps)
+        s.synchronize()
+        t_tuned += time.perf_counter() - t0
+        s.set_launch(256, 1)
         s.upload(X)
         s.initialize()
         s.step(args.warmup)
@@ -282,6 +296,11 @@ def run_c4(zdm, cases, args):
             "speedup_batched_vs_serial": round(t_serial / t_batch, 3),
             "serial_eager_s": round(t_eager, 4),
             "speedup_batched_vs_serial_eager": round(t_eager / t_batch, 3),
+            "serial_autotuned_s": round(t_tuned, 4),
+            "speedup_batched_vs_serial_autotuned": round(t_tuned / t_batch, 3),
+            "batched_launch": {"block_size": bs, "cells_per_thread": cpt,
+                               "autotune_ms_per_step": {"%dx%d" % k: round(v, 4) for k, v in tune.items()}},
+            "serial_autotuned_shapes": {"%dx%d" % k: v for k, v in sorted(shapes.items())},
             "dt_range": [min(dts), max(dts)],
             "step_bytes_per_cell": round(bpc, 1),
             "step_gbs": round(bpc * ncells * args.steps / t_batch / 1e9, 1),

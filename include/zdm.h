@@ -203,6 +203,17 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
  * counterpart in the reference, which has no differentiation path. */
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
                       const int* gauges, const double* obs, int exact_limiter);
+/* Launch shape of the transport kernels (k_prep, k_flux, k_post): workgroup size 64, 128 or
+ * 256 and 1..8 cells per thread; ensembles keep whole workgroups per member.  Results are
+ * bitwise the same for every shape.  Invalidates a step captured by zdm_step_graph.  No
+ * counterpart in the reference. */
+int zdm_set_launch(zdm_solver* s, int block_size, int cells_per_thread);
+int zdm_get_launch(zdm_solver* s, int* block_size, int* cells_per_thread);
+/* Autotuner: times `steps` steps of each candidate shape, {256, 128, 64} threads x {1, 2, 4}
+ * cells per thread (in that order, cells-per-thread outer), on the resident state, restores
+ * the state, aux and time, and keeps the fastest.  ms (may be NULL) receives the 9 ms/step.
+ * Call it before running (steps past t_end do nothing). */
+int zdm_autotune(zdm_solver* s, int steps, int* block_size, int* cells_per_thread, double* ms);
 /* MOOD overrides so far (cells of h < -1e-10 replaced by their first-order update), summed
  * over ensemble members; reset by zdm_initialize.  The reference prints this count per step. */
 int zdm_troubled_cells(zdm_solver* s, long long* count);

@@ -296,6 +296,7 @@ SOLVER_SYMBOLS = [
     "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres", "zdm_batch_create", "zdm_batch_time",
     "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
     "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout", "zdm_troubled_cells",
+    "zdm_set_launch", "zdm_get_launch", "zdm_autotune",
 ]
 
 
@@ -352,6 +353,10 @@ def model_lib(model):
     lib.zdm_adjoint_residuals.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_layout.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.zdm_troubled_cells.argtypes = [V, ctypes.POINTER(ctypes.c_longlong)]
+    lib.zdm_set_launch.argtypes = [V, ctypes.c_int, ctypes.c_int]
+    lib.zdm_get_launch.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.zdm_autotune.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                 _c_dbl_p]
     K = ctypes.POINTER(KrylovOptions)
     lib.zdm_chorin_setup.argtypes = [V, K]
     lib.zdm_chorin_step.argtypes = [V, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
@@ -560,6 +565,26 @@ class Solver:
         self._check(self.lib.zdm_adjoint_setup(self._h, int(nsteps), int(checkpoint_every),
                                                int(obs_every), len(g), g.ctypes.data_as(_c_int_p),
                                                _dp(o) if o.size else None, int(bool(exact_limiter))))
+
+    AUTOTUNE_CANDIDATES = [(256, 1), (128, 1), (64, 1), (256, 2), (128, 2), (64, 2), (256, 4), (128, 4),
+                           (64, 4)]
+
+    def set_launch(self, block_size, cells_per_thread=1):
+        """Workgroup size (64/128/256) and cells per thread of the transport kernels."""
+        self._check(self.lib.zdm_set_launch(self._h, int(block_size), int(cells_per_thread)))
+
+    def launch(self):
+        b, c = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.zdm_get_launch(self._h, ctypes.byref(b), ctypes.byref(c)))
+        return b.value, c.value
+
+    def autotune(self, steps=3):
+        """Time each candidate launch shape on the resident state (restored afterwards), keep
+        the fastest; returns (block_size, cells_per_thread, {shape: ms per step})."""
+        b, c = ctypes.c_int(0), ctypes.c_int(0)
+        ms = np.zeros(9)
+        self._check(self.lib.zdm_autotune(self._h, int(steps), ctypes.byref(b), ctypes.byref(c), _dp(ms)))
+        return b.value, c.value, dict(zip(self.AUTOTUNE_CANDIDATES, ms.tolist()))
 
     def troubled_cells(self):
         """MOOD overrides since the last initialize (summed over ensemble members)."""

@@ -167,6 +167,8 @@ struct DevMesh {
     const int* blk_first;
     const int* cell_off;
     int nmem;
+    int cpt;          // cells per thread of the transport kernels: thread t of a workgroup takes
+                      // cells first + t + j * blockDim, j < cpt
 };
 
 __device__ __forceinline__ int logical_block() {
@@ -179,22 +181,47 @@ __device__ __forceinline__ int logical_block() {
 }
 __device__ __forceinline__ int cell_index() { return logical_block() * blockDim.x + threadIdx.x; }
 
-// cell, logical block and ensemble member of this thread; c = ncells past a member's end
+// first cell, logical block and ensemble member of this thread (c = ncells past the member's
+// end).  MC (cells per thread > 1): cell(j) = its j-th cell, j < cpt, or ncells past the end;
+// without MC only cell(0) = c exists and the index math is that of one cell per thread.
+template <bool MC>
 struct Loc {
-    int c, lb, m;
+    int c, lb, m, base, end, nc;
+    __device__ __forceinline__ int cell(int j) const {
+        if constexpr (!MC) return c;
+        const int cj = base + j * (int)blockDim.x;
+        return cj < end ? cj : nc;
+    }
 };
-__device__ __forceinline__ Loc locate(const DevMesh& m) {
-    Loc L;
+template <bool MC = false>
+__device__ __forceinline__ Loc<MC> locate(const DevMesh& m) {
+    Loc<MC> L;
     L.lb = logical_block();
-    if (m.nmem == 0) {
-        L.m = 0;
-        L.c = L.lb * blockDim.x + threadIdx.x;
+    if constexpr (!MC) {
+        if (m.nmem == 0) {
+            L.m = 0;
+            L.c = L.lb * blockDim.x + threadIdx.x;
+            return L;
+        }
+        L.m = m.blk_member[L.lb];
+        const int c = m.cell_off[L.m] + (L.lb - m.blk_first[L.m]) * blockDim.x + threadIdx.x;
+        L.c = c < m.cell_off[L.m + 1] ? c : m.ncells;
+        return L;
+    } else {
+        L.nc = m.ncells;
+        const int span = (int)blockDim.x * m.cpt;
+        if (m.nmem == 0) {
+            L.m = 0;
+            L.base = L.lb * span + threadIdx.x;
+            L.end = m.ncells;
+        } else {
+            L.m = m.blk_member[L.lb];
+            L.base = m.cell_off[L.m] + (L.lb - m.blk_first[L.m]) * span + threadIdx.x;
+            L.end = m.cell_off[L.m + 1];
+        }
+        L.c = L.cell(0);
         return L;
     }
-    L.m = m.blk_member[L.lb];
-    const int c = m.cell_off[L.m] + (L.lb - m.blk_first[L.m]) * blockDim.x + threadIdx.x;
-    L.c = c < m.cell_off[L.m + 1] ? c : m.ncells;
-    return L;
 }
 
 template <int N>
@@ -482,22 +509,32 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
     }
 }
 
-template <int MAXF, int GRAD>
+template <int MAXF, int GRAD, bool MC>
 __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 const T* __restrict__ X, const T* __restrict__ Ag,
                                                 T* __restrict__ As, T* __restrict__ QF, int use_limiters,
                                                 T wet_eps, int zs_theta) {
-    const Loc L = locate(m);
+    const auto L = locate<MC>(m);
     ctrl += L.m;
     if (!ctrl->active) return;
-    const int c = L.c;
-    if (c >= m.ncells) return;
     const int ld = m.ld;
-    T ac[NA];
-    recon_cell<MAXF, GRAD>(
-        m, P, X, Ag, c, use_limiters, ac, [&](int k, int i, T v) { QF[((size_t)k * NQ + i) * ld + c] = v; },
-        wet_eps, zs_theta);
-    store_soa<NA>(As, ld, c, ac);
+    auto body = [&](int c) {
+        T ac[NA];
+        recon_cell<MAXF, GRAD>(
+            m, P, X, Ag, c, use_limiters, ac, [&](int k, int i, T v) { QF[((size_t)k * NQ + i) * ld + c] = v; },
+            wet_eps, zs_theta);
+        store_soa<NA>(As, ld, c, ac);
+    };
+    if constexpr (MC) {
+        for (int j = 0; j < m.cpt; ++j) {
+            const int c = L.cell(j);
+            if (c >= m.ncells) return;
+            body(c);
+        }
+    } else {
+        if (L.c >= m.ncells) return;
+        body(L.c);
+    }
 }
 
 // ------------------------------------------------------------------------------ k_flux
@@ -604,44 +641,54 @@ __device__ __forceinline__ void stage_update(const Ctrl* ctrl, int c, int ld, co
     }
 }
 
-template <int MAXF, int MODE>
+template <int MAXF, int MODE, bool MC>
 __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 int stage, const T* __restrict__ QF,
                                                 const T* __restrict__ As, const T* In, T* Out,
                                                 const T* __restrict__ Ag, int wb, const T* X0) {
-    const Loc L = locate(m);
+    const auto L = locate<MC>(m);
     ctrl += L.m;
     if (!ctrl->active) return;
-    const int c = L.c;
-    if (c >= m.ncells) return;
     const int ld = m.ld;
-    const T time = ctrl->stage_t[stage];
-    T fc[NQ], ac[NA];
-    load_soa<NA>(As, ld, c, ac);
-    T qcell[NQ];   // clamped cell state (only read by the well-balanced term)
-    if constexpr (HasCellSource<T>::value) {
-        if (wb) {
-            T a0[NA];
-            load_soa<NQ>(In, ld, c, qcell);
-            load_soa<NA>(Ag, ld, c, a0);
-            auto rq = M::update_variables(qcell, a0, P.p, 0.0);
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
+    auto body = [&](int c) {
+        const T time = ctrl->stage_t[stage];
+        T fc[NQ], ac[NA];
+        load_soa<NA>(As, ld, c, ac);
+        T qcell[NQ];   // clamped cell state (only read by the well-balanced term)
+        if constexpr (HasCellSource<T>::value) {
+            if (wb) {
+                T a0[NA];
+                load_soa<NQ>(In, ld, c, qcell);
+                load_soa<NA>(Ag, ld, c, a0);
+                auto rq = M::update_variables(qcell, a0, P.p, 0.0);
+    #pragma unroll
+                for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
+            }
         }
+        flux_cell<MAXF>(
+            m, P, c, time, ac, wb, qcell,
+            [&](int k, T* qo) {
+    #pragma unroll
+                for (int i = 0; i < NQ; ++i) qo[i] = QF[((size_t)k * NQ + i) * ld + c];
+            },
+            [&](int k, int nb, int ks, T* qn, T* an) {
+    #pragma unroll
+                for (int i = 0; i < NQ; ++i) qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
+                load_soa<NA>(As, ld, nb, an);
+            },
+            fc);
+        stage_update<MODE>(ctrl, c, ld, fc, In, Out, X0);
+    };
+    if constexpr (MC) {
+        for (int j = 0; j < m.cpt; ++j) {
+            const int c = L.cell(j);
+            if (c >= m.ncells) return;
+            body(c);
+        }
+    } else {
+        if (L.c >= m.ncells) return;
+        body(L.c);
     }
-    flux_cell<MAXF>(
-        m, P, c, time, ac, wb, qcell,
-        [&](int k, T* qo) {
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) qo[i] = QF[((size_t)k * NQ + i) * ld + c];
-        },
-        [&](int k, int nb, int ks, T* qn, T* an) {
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
-            load_soa<NA>(As, ld, nb, an);
-        },
-        fc);
-    stage_update<MODE>(ctrl, c, ld, fc, In, Out, X0);
 }
 
 // ------------------------------------------------------------------------------ source
@@ -787,16 +834,15 @@ __device__ __forceinline__ T block_min(T v) {
 // POST: 0 = CFL only, 1 = constraints + CFL, 2 = constraints + source + constraints + CFL,
 //       3 = source only (no CFL), 4 = constraints only (no CFL),
 //       5 = source with the step's dt + constraints (no CFL; the Chorin predictor)
-template <int POST>
+template <int POST, bool MC>
 __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl* __restrict__ ctrl, T* X,
                                                 T* A, T* __restrict__ partials, T dt_override) {
-    const Loc L = locate(m);
+    const auto L = locate<MC>(m);
     ctrl += L.m;
     if (!ctrl->active) return;
-    const int c = L.c;
     const int ld = m.ld;
     T ratio = DBL_MAX;
-    if (c < m.ncells) {
+    auto body = [&](int c) -> T {
         T q[NQ], a[NA];
         load_soa<NQ>(X, ld, c, q);
         load_soa<NA>(A, ld, c, a);
@@ -810,7 +856,18 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
             store_soa<NQ>(X, ld, c, q);
             store_soa<NA>(A, ld, c, a);
         }
-        if constexpr (POST <= 2) ratio = cfl_ratio(P, q, a, m.rc[c]);
+        if constexpr (POST <= 2) return cfl_ratio(P, q, a, m.rc[c]);
+        return (T)DBL_MAX;
+    };
+    if constexpr (MC) {
+        for (int j = 0; j < m.cpt; ++j) {
+            const int c = L.cell(j);
+            if (c >= m.ncells) break;
+            const T r = body(c);
+            ratio = r < ratio ? r : ratio;
+        }
+    } else {
+        if (L.c < m.ncells) ratio = body(L.c);
     }
     if constexpr (POST <= 2) {
         ratio = block_min(ratio);
@@ -825,17 +882,16 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
 // 363-381, at the step's start time), then constraints again; then [source, constraints,] CFL as
 // k_post.  Cells are independent: Xs and As are read-only here, X and A are written per cell.
 // Untroubled cells get one more clamp in the reference, a no-op (the clamp is idempotent).
-template <int MAXF, int SRC>
+template <int MAXF, int SRC, bool MC>
 __global__ __launch_bounds__(BLOCK) void k_post_mood(DevMesh m, Params P, Ctrl* __restrict__ ctrl, T* X,
                                                      T* A, const T* __restrict__ Xs,
                                                      const T* __restrict__ As, T* __restrict__ partials) {
-    const Loc L = locate(m);
+    const auto L = locate<MC>(m);
     ctrl += L.m;
     if (!ctrl->active) return;
-    const int c = L.c;
     const int ld = m.ld;
     T ratio = DBL_MAX;
-    if (c < m.ncells) {
+    auto body = [&](int c) -> T {
         T q[NQ], a[NA];
         load_soa<NQ>(X, ld, c, q);
         load_soa<NA>(A, ld, c, a);
@@ -871,7 +927,17 @@ __global__ __launch_bounds__(BLOCK) void k_post_mood(DevMesh m, Params P, Ctrl* 
         }
         store_soa<NQ>(X, ld, c, q);
         store_soa<NA>(A, ld, c, a);
-        ratio = cfl_ratio(P, q, a, m.rc[c]);
+        return cfl_ratio(P, q, a, m.rc[c]);
+    };
+    if constexpr (MC) {
+        for (int j = 0; j < m.cpt; ++j) {
+            const int c = L.cell(j);
+            if (c >= m.ncells) break;
+            const T r = body(c);
+            ratio = r < ratio ? r : ratio;
+        }
+    } else {
+        if (L.c < m.ncells) ratio = body(L.c);
     }
     ratio = block_min(ratio);
     if (threadIdx.x == 0) partials[L.lb] = ratio;
@@ -881,19 +947,18 @@ __global__ __launch_bounds__(BLOCK) void k_post_mood(DevMesh m, Params P, Ctrl* 
 // intermediate states are kept: reads the final stage's output X1, writes the source's solution
 // X3 and the aux A3 it saw (POST 2), the end-of-step state Xout and its aux A, and the CFL
 // partial minima; the same operations as k_post.  dt_log (may be null) receives the step's dt.
-template <int POST>
+template <int POST, bool MC>
 __global__ __launch_bounds__(BLOCK) void k_post_record(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                        const T* __restrict__ X1, T* __restrict__ Xout, T* A,
                                                        T* __restrict__ X3, T* __restrict__ A3,
                                                        T* __restrict__ partials, T* __restrict__ dt_log) {
-    const Loc L = locate(m);
+    const auto L = locate<MC>(m);
     ctrl += L.m;
     if (!ctrl->active) return;
     if (dt_log && L.lb == 0 && threadIdx.x == 0) *dt_log = ctrl->dt;   // the step's dt
-    const int c = L.c;
     const int ld = m.ld;
     T ratio = DBL_MAX;
-    if (c < m.ncells) {
+    auto body = [&](int c) -> T {
         T q[NQ], a[NA];
         load_soa<NQ>(X1, ld, c, q);
         load_soa<NA>(A, ld, c, a);
@@ -906,7 +971,17 @@ __global__ __launch_bounds__(BLOCK) void k_post_record(DevMesh m, Params P, cons
         }
         store_soa<NQ>(Xout, ld, c, q);
         store_soa<NA>(A, ld, c, a);
-        ratio = cfl_ratio(P, q, a, m.rc[c]);
+        return cfl_ratio(P, q, a, m.rc[c]);
+    };
+    if constexpr (MC) {
+        for (int j = 0; j < m.cpt; ++j) {
+            const int c = L.cell(j);
+            if (c >= m.ncells) break;
+            const T r = body(c);
+            ratio = r < ratio ? r : ratio;
+        }
+    } else {
+        if (L.c < m.ncells) ratio = body(L.c);
     }
     ratio = block_min(ratio);
     if (threadIdx.x == 0) partials[L.lb] = ratio;
@@ -1008,6 +1083,11 @@ struct zdm_solver {
     zdm_options opt;
     Params P;
     int ncells, nfaces, ld, maxf, nblocks;
+    int bs;                           // workgroup size of the transport kernels (64, 128, 256)
+    int ncb;                          // workgroups of 256 over all cells (per-cell kernels via cell_index)
+    int maxblocks;                    // block tables / CFL partials capacity (bs = 64, cpt = 1)
+    std::vector<int> coff_h;          // ensemble cell offsets (host copy)
+    std::vector<double> tune_ms;      // last autotune: ms per step of each candidate
     hipStream_t stream;
     char* arena;
     size_t arena_bytes;
@@ -1033,6 +1113,25 @@ struct zdm_solver {
 namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Workgroups of the transport kernels for workgroup size bs and cpt cells per thread: one
+// range for a single mesh, whole workgroups per member for an ensemble (member tables).
+// Returns the workgroup count.
+int launch_tables(int nc, const std::vector<int>& coff, int bs, int cpt, std::vector<int>& blk_member,
+                  std::vector<int>& blk_first) {
+    const int span = bs * cpt;
+    blk_member.clear();
+    blk_first.clear();
+    if (coff.empty()) return (nc + span - 1) / span;
+    const int nmem = (int)coff.size() - 1;
+    blk_first.assign(nmem + 1, 0);
+    for (int i = 0; i < nmem; ++i) {
+        const int cnt = coff[i + 1] - coff[i];
+        blk_first[i + 1] = blk_first[i] + (cnt + span - 1) / span;
+        for (int b = blk_first[i]; b < blk_first[i + 1]; ++b) blk_member.push_back(i);
+    }
+    return blk_first[nmem];
+}
 
 // Least-squares weights of LeastSquaresGradient::Setup (Gradient.hpp:222-298), order 1
 bool chol_solve_host(int n, std::vector<T>& A, std::vector<T>& b) {
@@ -1062,41 +1161,56 @@ bool chol_solve_host(int n, std::vector<T>& A, std::vector<T>& b) {
     return true;
 }
 
-template <int MAXF>
-void launch_prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* As, T* QF) {
-    dim3 g(s->nblocks), b(BLOCK);
+template <int MAXF, bool MC>
+void launch_prep_t(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* As, T* QF) {
+    dim3 g(s->nblocks), b(s->bs);
     const int lim = s->opt.use_limiters, pos = s->opt.positivity;
     const T eps = s->opt.wet_dry_eps;
     if (s->opt.order < 2)
-        k_prep<MAXF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+        k_prep<MAXF, 0, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
     else if (pos != 0)
-        k_prep<MAXF, 3><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, pos == 2);
+        k_prep<MAXF, 3, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, pos == 2);
     else if (s->opt.gradient == 1)
-        k_prep<MAXF, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+        k_prep<MAXF, 2, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
     else
-        k_prep<MAXF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+        k_prep<MAXF, 1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+}
+template <int MAXF>
+void launch_prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* As, T* QF) {
+    if (s->dm.cpt > 1)
+        launch_prep_t<MAXF, true>(s, ctrl, Xin, Ag, As, QF);
+    else
+        launch_prep_t<MAXF, false>(s, ctrl, Xin, Ag, As, QF);
 }
 
 // MODE_FINAL: Out = al In + be F(In) + ga X0 (X0 = Out in place)
-template <int MAXF>
-void launch_flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag,
+template <int MAXF, bool MC>
+void launch_flux_t(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag,
                  const T* QF, const T* As, const T* X0) {
-    dim3 g(s->nblocks), b(BLOCK);
+    dim3 g(s->nblocks), b(s->bs);
     const int wb = s->opt.well_balanced;
     switch (mode) {
         case MODE_RHS:
-            k_flux<MAXF, MODE_RHS><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_RHS, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
         case MODE_EULER:
-            k_flux<MAXF, MODE_EULER><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_EULER, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
         case MODE_STAGE:
-            k_flux<MAXF, MODE_STAGE><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_STAGE, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
         default:
-            k_flux<MAXF, MODE_FINAL><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_FINAL, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
     }
+}
+template <int MAXF>
+void launch_flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* Out, const T* Ag,
+                 const T* QF, const T* As, const T* X0) {
+    if (s->dm.cpt > 1)
+        launch_flux_t<MAXF, true>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
+    else
+        launch_flux_t<MAXF, false>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
 }
 
 // face states / clamped aux into (QF, As): the solver's own buffers unless given
@@ -1126,13 +1240,19 @@ void flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* 
 
 // post-stage with the MOOD override (positivity "mood"): reads the saved state s->Xs
 void post_mood(zdm_solver* s) {
-    dim3 g(s->nblocks), b(BLOCK);
+    dim3 g(s->nblocks), b(s->bs);
     const bool src = s->opt.source;
-#define ZDM_MOOD(MF)                                                                                                 \
+#define ZDM_MOOD1(MF, MC)                                                                                            \
     if (src)                                                                                                         \
-        k_post_mood<MF, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->Xs, s->As, s->partials);   \
+        k_post_mood<MF, 1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->Xs, s->As, s->partials); \
     else                                                                                                             \
-        k_post_mood<MF, 0><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->Xs, s->As, s->partials);
+        k_post_mood<MF, 0, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->Xs, s->As, s->partials);
+#define ZDM_MOOD(MF)                                                                                                 \
+    if (s->dm.cpt > 1) {                                                                                             \
+        ZDM_MOOD1(MF, true)                                                                                          \
+    } else {                                                                                                         \
+        ZDM_MOOD1(MF, false)                                                                                         \
+    }
     if (s->maxf <= 2) {
         ZDM_MOOD(2)
     } else if (s->maxf == 3) {
@@ -1141,19 +1261,27 @@ void post_mood(zdm_solver* s) {
         ZDM_MOOD(4)
     }
 #undef ZDM_MOOD
+#undef ZDM_MOOD1
 }
 
-void post(zdm_solver* s, int mode, T dt_override = 0) {
-    dim3 g(s->nblocks), b(BLOCK);
+template <bool MC>
+void post_t(zdm_solver* s, int mode, T dt_override) {
+    dim3 g(s->nblocks), b(s->bs);
     switch (mode) {
-        case 5: k_post<5><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
-        case 6: k_post<4><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
-        case 0: k_post<0><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
-        case 1: k_post<1><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
-        case 2: k_post<2><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
-        case 3: k_post<3><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl_rhs, s->X, s->A, s->partials, dt_override); break;
-        default: k_post<4><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl_rhs, s->X, s->A, s->partials, dt_override); break;
+        case 5: k_post<5, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
+        case 6: k_post<4, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
+        case 0: k_post<0, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
+        case 1: k_post<1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
+        case 2: k_post<2, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
+        case 3: k_post<3, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl_rhs, s->X, s->A, s->partials, dt_override); break;
+        default: k_post<4, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl_rhs, s->X, s->A, s->partials, dt_override); break;
     }
+}
+void post(zdm_solver* s, int mode, T dt_override = 0) {
+    if (s->dm.cpt > 1)
+        post_t<true>(s, mode, dt_override);
+    else
+        post_t<false>(s, mode, dt_override);
 }
 
 void finalize(zdm_solver* s, int what) {
@@ -1273,20 +1401,16 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     s->nfaces = nf;
     s->maxf = MF;
     s->ld = (int)align_up((size_t)nc, 256);
-    s->nblocks = (nc + BLOCK - 1) / BLOCK;
     s->nmem = nmem;
     s->nctrl = nmem > 0 ? nmem : 1;
+    s->ncb = (nc + BLOCK - 1) / BLOCK;
+    s->bs = BLOCK;
+    s->dm.cpt = 1;
     std::vector<int> blk_member, blk_first, coff;
-    if (nmem > 0) {
-        blk_first.assign(nmem + 1, 0);
-        coff.assign(cell_off, cell_off + nmem + 1);
-        for (int i = 0; i < nmem; ++i) {
-            const int cnt = coff[i + 1] - coff[i];
-            blk_first[i + 1] = blk_first[i] + (cnt + BLOCK - 1) / BLOCK;
-            for (int b = blk_first[i]; b < blk_first[i + 1]; ++b) blk_member.push_back(i);
-        }
-        s->nblocks = blk_first[nmem];
-    }
+    if (nmem > 0) coff.assign(cell_off, cell_off + nmem + 1);
+    s->coff_h = coff;
+    s->maxblocks = launch_tables(nc, coff, 64, 1, blk_member, blk_first);
+    s->nblocks = launch_tables(nc, coff, BLOCK, 1, blk_member, blk_first);
     const int ld = s->ld;
     const int MFa = MF <= 2 ? 2 : MF;
 
@@ -1381,10 +1505,10 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     size_t oX = take(sQ), oW = take(sQ), oF = take(sQ), oA = take(sA), oAs = take(sA), oAt = take(sA);
     size_t oQF = take(sizeof(T) * (size_t)MFa * NQ * ld);
     size_t oXs = take(s->opt.positivity == 1 ? sQ : 8);   // MOOD: the saved start-of-step state
-    size_t oPart = take(sizeof(T) * (size_t)s->nblocks);
+    size_t oPart = take(sizeof(T) * (size_t)s->maxblocks);
     size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
     size_t oCtrl = take(sizeof(Ctrl) * s->nctrl), oCtrlR = take(sizeof(Ctrl) * s->nctrl);
-    size_t oBm = take(sizeof(int) * std::max<size_t>(blk_member.size(), 1)),
+    size_t oBm = take(sizeof(int) * std::max<size_t>(nmem > 0 ? (size_t)s->maxblocks : 0, 1)),
            oBf = take(sizeof(int) * std::max<size_t>(blk_first.size(), 1)),
            oCo = take(sizeof(int) * std::max<size_t>(coff.size(), 1));
     size_t oVol = take(sizeof(T) * ld), oCx = take(sizeof(T) * ld), oCy = take(sizeof(T) * ld), oRc = take(sizeof(T) * ld);
@@ -2046,6 +2170,105 @@ int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_e
     if (!s) return fail("null solver");
     if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs)) return -1;
     s->adj->freeze_limiter = exact_limiter ? 0 : 1;
+    return 0;
+}
+
+int zdm_set_launch(zdm_solver* s, int block_size, int cells_per_thread) {
+    if (!s) return fail("null solver");
+    if (block_size != 64 && block_size != 128 && block_size != 256)
+        return fail("zdm_set_launch: block_size must be 64, 128 or 256");
+    if (cells_per_thread < 1 || cells_per_thread > 8) return fail("zdm_set_launch: cells_per_thread must be 1..8");
+    std::vector<int> bm, bf;
+    const int nb = launch_tables(s->ncells, s->coff_h, block_size, cells_per_thread, bm, bf);
+    if (nb > s->maxblocks) return fail("zdm_set_launch: too many workgroups");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (!bm.empty()) {
+        HIPCHK(hipMemcpy(const_cast<int*>(s->dm.blk_member), bm.data(), sizeof(int) * bm.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(const_cast<int*>(s->dm.blk_first), bf.data(), sizeof(int) * bf.size(), hipMemcpyHostToDevice));
+    }
+    s->bs = block_size;
+    s->dm.cpt = cells_per_thread;
+    s->nblocks = nb;
+    // a captured step holds the old launch shapes
+    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+    return 0;
+}
+
+int zdm_get_launch(zdm_solver* s, int* block_size, int* cells_per_thread) {
+    if (!s) return fail("null solver");
+    if (block_size) *block_size = s->bs;
+    if (cells_per_thread) *cells_per_thread = s->dm.cpt;
+    return 0;
+}
+
+int zdm_autotune(zdm_solver* s, int steps, int* block_size, int* cells_per_thread, double* ms) {
+    if (!s) return fail("null solver");
+    if (steps < 1) return fail("zdm_autotune: steps must be >= 1");
+    static const int cand[9][2] = {{256, 1}, {128, 1}, {64, 1}, {256, 2}, {128, 2}, {64, 2}, {256, 4}, {128, 4}, {64, 4}};
+    const size_t bX = sizeof(T) * (size_t)NQ * s->ld, bA = sizeof(T) * (size_t)NA * s->ld,
+                 bC = sizeof(Ctrl) * (size_t)s->nctrl;
+    char* save = nullptr;
+    HIPCHK(hipMalloc(&save, bX + bA + bC));
+    auto copy3 = [&](bool out) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(out ? save : (char*)s->X, out ? (char*)s->X : save, bX, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out ? save + bX : (char*)s->A, out ? (char*)s->A : save + bX, bA, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out ? save + bX + bA : (char*)s->ctrl, out ? (char*)s->ctrl : save + bX + bA, bC,
+                               hipMemcpyDeviceToDevice, s->stream);
+        return e;
+    };
+    const bool timing = s->timing;
+    s->timing = false;
+    int best = 0;
+    double best_ms = 0;
+    s->tune_ms.assign(9, 0.0);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (copy3(true) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail("zdm_autotune: setup failed");
+    for (int k = 0; k < 9 && rc == 0; ++k) {
+        if (zdm_set_launch(s, cand[k][0], cand[k][1])) {
+            rc = -1;
+            break;
+        }
+        float t = 0;
+        hipError_t e = copy3(false);
+        if (e == hipSuccess) {
+            enqueue_step(s);   // warm-up
+            e = copy3(false);
+        }
+        if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+        for (int i = 0; i < steps && e == hipSuccess; ++i) enqueue_step(s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) {
+            rc = fail(std::string("zdm_autotune: ") + hipGetErrorString(e));
+            break;
+        }
+        s->tune_ms[k] = (double)t / steps;
+        if (k == 0 || s->tune_ms[k] < best_ms) {
+            best_ms = s->tune_ms[k];
+            best = k;
+        }
+    }
+    if (rc == 0) rc = zdm_set_launch(s, cand[best][0], cand[best][1]);
+    if (rc == 0 && (copy3(false) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess))
+        rc = fail("zdm_autotune: restoring the state failed");
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(save);
+    s->timing = timing;
+    if (rc) return rc;
+    if (block_size) *block_size = cand[best][0];
+    if (cells_per_thread) *cells_per_thread = cand[best][1];
+    if (ms)
+        for (int k = 0; k < 9; ++k) ms[k] = s->tune_ms[k];
     return 0;
 }
 
