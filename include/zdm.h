@@ -196,8 +196,10 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
  *   -1  the same without the face states (the reverse sweep recomputes them per stage);
  *    1  the state before every step (each step's stages are re-run in the reverse sweep);
  *    k  a checkpoint every k steps (each segment is re-run once more);
+ *  -2-c (c >= 1) c state slots, checkpoints placed by the optimal binomial schedule
+ *       (Griewank-Walther "revolve": the fewest forward re-steps for c slots);
  *    0  the first of -2, -1 that fits in half of the free device memory, else 1 if the
- *       states fit in a quarter of it, else k = ceil(sqrt(nsteps)).
+ *       states fit in a quarter of it, else the binomial schedule on the slots that do.
  * Every layout gives bitwise the same gradient.  Order 1, or order 2 with
  * the Green-Gauss gradient; not for ensembles or the well-balanced extension.  No
  * counterpart in the reference, which has no differentiation path. */
@@ -217,8 +219,9 @@ int zdm_autotune(zdm_solver* s, int steps, int* block_size, int* cells_per_threa
 /* MOOD overrides so far (cells of h < -1e-10 replaced by their first-order update), summed
  * over ensemble members; reset by zdm_initialize.  The reference prints this count per step. */
 int zdm_troubled_cells(zdm_solver* s, long long* count);
-/* The layout zdm_adjoint_setup chose: checkpoint interval (1 with a tape) and tape level
- * (0 none, 1 stage states, 2 stage and face states). */
+/* The layout zdm_adjoint_setup chose: checkpoint_every in its own encoding (1 with a tape,
+ * -2-c for the binomial schedule on c slots) and tape level (0 none, 1 stage states, 2 stage
+ * and face states). */
 int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape);
 /* Forward run (X0, A0 in the zdm_state_upload layout, A0 may be NULL).  dts = NULL takes
  * dt = CFL each step and records it; otherwise the given sequence is used (a frozen-dt

@@ -80,10 +80,12 @@ def test_adjoint_vs_central_differences_on_device():
 @pytest.mark.parametrize("order", [1, 2])
 def test_trajectory_layout_does_not_change_the_gradient(order):
     """Tape with face states (-2), tape of stage states (-1), every state (1), checkpoints
-    (3, 8) and the automatic choice (0): bitwise the same gradient, since what a tape keeps is
-    what the re-runs recompute."""
+    (3, 8), the binomial schedule on 1, 2, 3 and 5 slots (-3, -4, -5, -7) and the automatic
+    choice (0): bitwise the same gradient, since what a tape keeps is what the re-runs
+    recompute."""
     grads = []
-    for ck, want in ((-2, (1, 2)), (-1, (1, 1)), (0, None), (1, (1, 0)), (3, (3, 0)), (8, (8, 0))):
+    for ck, want in ((-2, (1, 2)), (-1, (1, 1)), (0, None), (1, (1, 0)), (3, (3, 0)), (8, (8, 0)),
+                     (-3, (-3, 0)), (-4, (-4, 0)), (-5, (-5, 0)), (-7, (-7, 0))):
         s, mesh, X, A, bc, gauges, obs = make(order, ck=ck)
         if want is not None:
             assert s.adjoint_layout() == want
@@ -91,6 +93,19 @@ def test_trajectory_layout_does_not_change_the_gradient(order):
         grads.append(s.adjoint_gradient())
         s.close()
     assert np.all(np.isfinite(grads[0][0])) and np.abs(grads[0][0]).max() > 0
+    for g in grads[1:]:
+        assert np.array_equal(g[0], grads[0][0]) and np.array_equal(g[1], grads[0][1])
+
+
+def test_binomial_schedule_on_a_longer_trajectory():
+    """21 steps on 3 and 4 slots (split by the dynamic programme, nested several levels deep):
+    bitwise the gradient of the tape."""
+    grads = []
+    for ck in (-2, -5, -6):
+        s, mesh, X, A, bc, gauges, obs = make(2, steps=21, ck=ck)
+        s.adjoint_forward(X, A)
+        grads.append(s.adjoint_gradient())
+        s.close()
     for g in grads[1:]:
         assert np.array_equal(g[0], grads[0][0]) and np.array_equal(g[1], grads[0][1])
 

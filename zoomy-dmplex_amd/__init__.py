@@ -555,7 +555,8 @@ class Solver:
         exact_limiter=False holds each face's limiter value fixed (frozen-limiter
         linearisation); True differentiates it through its active branch.
         checkpoint_every: -2 / -1 tape with / without face states, 1 every state, k > 1 a
-        checkpoint every k steps, 0 the richest layout that fits in device memory."""
+        checkpoint every k steps, -2-c (c >= 1) the binomial schedule on c state slots, 0 the
+        richest layout that fits in device memory."""
         g = np.ascontiguousarray(gauges, dtype=np.int32)
         nobs = nsteps // obs_every if len(g) and obs_every else 0
         o = np.zeros((nobs, len(g))) if obs is None else np.ascontiguousarray(obs, dtype=np.float64)

@@ -1067,6 +1067,7 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
 }
 
 #include "krylov.inc"
+#include "binomial.h"
 #include "adjoint.inc"
 #ifdef ZOOMY_MODEL_IS_VAM
 #include "chorin.inc"
@@ -2284,7 +2285,7 @@ int zdm_troubled_cells(zdm_solver* s, long long* count) {
 
 int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape) {
     if (!s || !s->adj) return fail("zdm_adjoint_layout: call zdm_adjoint_setup first");
-    if (checkpoint_every) *checkpoint_every = s->adj->ck;
+    if (checkpoint_every) *checkpoint_every = s->adj->slots > 0 ? -2 - s->adj->slots : s->adj->ck;
     if (tape) *tape = s->adj->tape;
     return 0;
 }
