@@ -376,7 +376,8 @@ def run_c5(zdm, cases, args):
             "grad_norm_bed": float(np.linalg.norm(gX[:, 0])),
             "grad_norm_manning": float(np.linalg.norm(gA[:, 1])),
             "checkpoint_every": ck,
-            "tape": ["states or checkpoints", "stage states", "stage and face states"][tape],
+            "tape": ("binomial schedule on %d state slots" % (-2 - ck) if ck < -2 else
+                     ["states or checkpoints", "stage states", "stage and face states"][tape]),
         },
         "roofline": None,
     }
@@ -395,7 +396,8 @@ def main():
     ap.add_argument("--max-it", type=int, default=400, help="c3 GMRES iteration cap")
     ap.add_argument("--checkpoint-every", type=int, default=0,
                     help="c5 trajectory layout: -2/-1 tape with/without face states, 1 every "
-                         "state, k>1 checkpoints, 0 the richest that fits")
+                         "state, k>1 checkpoints, -2-c binomial schedule on c state slots, 0 "
+                         "the richest that fits")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: c2 200, c3 10, c4 300, c5 500)")

@@ -15,6 +15,7 @@
 #pragma once
 #include <algorithm>
 #include <functional>
+#include <utility>
 #include <vector>
 
 namespace zdm_binomial {
@@ -48,37 +49,71 @@ inline std::vector<int> plan(int T, int C, std::vector<double>* cost_out = nullp
     return split;
 }
 
-// Reverse sweep over steps T-1 .. 0 with slots 0 .. C (slot 0 holds state 0 on entry).
-//   advance(from, to, l, m): run steps l .. m-1 starting from the state in slot `from` (state
-//     l) and leave state m in slot `to`; to = -1 names the scratch state outside the pool.
-//   adjoint(n, from): the adjoint of step n given its start state in slot `from` (-1: the
-//     scratch state).
-// Each callback returns 0 on success; the first failure ends the sweep and is returned.
-template <typename Advance, typename Adjoint>
-int reverse(int T, int C, Advance&& advance, Adjoint&& adjoint, double max_work = 3.0e8) {
-    const std::vector<int> split = plan(T, C, nullptr, max_work);
-    std::vector<int> freeslots;
-    for (int k = C; k >= 1; --k) freeslots.push_back(k);
-    std::function<int(int, int, int, int)> rev = [&](int l, int r, int sl, int c) -> int {
-        const int n = r - l;
-        if (n == 1) return adjoint(l, sl);
-        if (c == 0 || freeslots.empty()) {
-            for (int k = r - 1; k > l; --k) {
-                if (int e = advance(sl, -1, l, k)) return e;
-                if (int e = adjoint(k, -1)) return e;
-            }
-            return adjoint(l, sl);
-        }
+// The schedule for T steps on slots 0 .. C, with its split table.
+struct Schedule {
+    int T = 0, C = 0;
+    std::vector<int> split;
+
+    Schedule() = default;
+    Schedule(int T_, int C_, double max_work = 3.0e8) : T(T_), C(C_), split(plan(T_, C_, nullptr, max_work)) {}
+
+    // first stretch for n steps with c free slots
+    int stretch(int n, int c) const {
         const int want = split.empty() ? n / (c + 1) : split[(size_t)n * (C + 1) + c];
-        const int m = l + std::min(n - 1, std::max(1, want));
-        const int sm = freeslots.back();
-        freeslots.pop_back();
-        if (int e = advance(sl, sm, l, m)) return e;
-        if (int e = rev(m, r, sm, c - 1)) return e;
-        freeslots.push_back(sm);
-        return rev(l, m, sl, c);
-    };
-    return rev(0, T, 0, C);
-}
+        return std::min(n - 1, std::max(1, want));
+    }
+
+    // The first sweep: every advance the schedule makes before its first adjoint runs forward
+    // from state 0 without a break, through states the forward run computes anyway.  The
+    // (state, slot) pairs it leaves (slot -1: the scratch state) can be stored by the forward
+    // run itself; reverse(..., stored = true) then skips those advances, which saves exactly
+    // T - 1 steps on every schedule.
+    std::vector<std::pair<int, int>> first_sweep() const {
+        std::vector<std::pair<int, int>> out{{0, 0}};
+        int l = 0, c = C, slot = 1;
+        while (T - l > 1 && c > 0) {
+            l += stretch(T - l, c);
+            out.push_back({l, slot++});
+            --c;
+        }
+        if (T - l > 1) out.push_back({T - 1, -1});
+        return out;
+    }
+
+    // Reverse sweep over steps T-1 .. 0 with slots 0 .. C (slot 0 holds state 0 on entry; with
+    // stored, every pair of first_sweep() is in place too).
+    //   advance(from, to, l, m): run steps l .. m-1 starting from the state in slot `from`
+    //     (state l) and leave state m in slot `to`; to = -1 names the scratch state.
+    //   adjoint(n, from): the adjoint of step n given its start state in slot `from` (-1: the
+    //     scratch state).
+    // Each callback returns 0 on success; the first failure ends the sweep and is returned.
+    template <typename Advance, typename Adjoint>
+    int reverse(Advance&& advance, Adjoint&& adjoint, bool stored = false) const {
+        std::vector<int> freeslots;
+        for (int k = C; k >= 1; --k) freeslots.push_back(k);
+        // top: the stretch ends at T and its advances belong to the first sweep
+        std::function<int(int, int, int, int, bool)> rev = [&](int l, int r, int sl, int c, bool top) -> int {
+            const int n = r - l;
+            if (n == 1) return adjoint(l, sl);
+            if (c == 0 || freeslots.empty()) {
+                for (int k = r - 1; k > l; --k) {
+                    if (!(top && stored && k == r - 1))
+                        if (int e = advance(sl, -1, l, k)) return e;
+                    if (int e = adjoint(k, -1)) return e;
+                }
+                return adjoint(l, sl);
+            }
+            const int m = l + stretch(n, c);
+            const int sm = freeslots.back();
+            freeslots.pop_back();
+            if (!(top && stored))
+                if (int e = advance(sl, sm, l, m)) return e;
+            if (int e = rev(m, r, sm, c - 1, top)) return e;
+            freeslots.push_back(sm);
+            return rev(l, m, sl, c, false);
+        };
+        return rev(0, T, 0, C, true);
+    }
+};
 
 }  // namespace zdm_binomial
