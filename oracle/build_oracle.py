@@ -13,7 +13,7 @@ MODELS_DIR = os.path.join(ROOT, "zoomy-dmplex_amd", "models")
 MODELS = ["sme0_1d", "sme0", "sme1", "sme1m", "sme2", "vam"]
 
 
-TANGENT_MODELS = ["sme1m"]
+TANGENT_MODELS = ["sme1m", "vam"]
 
 
 def build(model, tangent=False):

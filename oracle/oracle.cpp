@@ -24,11 +24,11 @@
 #include <type_traits>
 #include <utility>
 
+#ifdef ORACLE_TANGENT
+#include "dual.H"   // before the model: its templates name std::sqrt / std::fabs on duals
+#endif
 #include "Numerics.H"
 #include "zdm_mesh.h"
-#ifdef ORACLE_TANGENT
-#include "dual.H"
-#endif
 
 template <typename TT, typename = void>
 struct HasCellSource : std::false_type {};
@@ -721,9 +721,7 @@ T run_loss(const zdm_mesh* m, const int* face_bc, const T* P, int order, int ssp
 #ifdef ZOOMY_MODEL_IS_VAM
 #include "ChorinOps.H"
 namespace {
-#ifndef ORACLE_TANGENT
 #include "oracle_chorin.inc"
-#endif
 }  // namespace
 #endif
 

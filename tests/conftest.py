@@ -226,6 +226,19 @@ class Oracle:
         return n, t[0], its.value, reason.value
 
 
+    def chorin_loss(self, mesh, face_bc, p, X0, A0, dts, gauges, obs_every, obs, order=1, use_limiters=1,
+                    restart=30, max_it=3000, rtol=1e-14, atol=1e-30):
+        """Calibration loss over Chorin steps with frozen dts (assembled pressure operator)."""
+        L = self.L
+        L.orc_chorin_loss.restype = ctypes.c_double
+        L.orc_chorin_loss.argtypes = [ctypes.POINTER(zdm._ZdmMesh), _ip, _dp, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, ctypes.c_int, _dp,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        g = np.ascontiguousarray(gauges, dtype=np.int32)
+        c = lambda a: dp(np.ascontiguousarray(a, dtype=np.float64))
+        return L.orc_chorin_loss(mesh.handle, ip(face_bc), dp(p), order, use_limiters, len(dts), c(dts), c(X0),
+                                 c(A0), len(g), ip(g), obs_every, c(obs), restart, max_it, rtol, atol)
+
     # ---- calibration loss with frozen time steps (liboracle_<model>.so)
     def loss(self, mesh, face_bc, p, X0, A0, dts, gauges, obs_every, obs, order=2, ssp_stages=2,
              use_limiters=1):
@@ -264,6 +277,23 @@ class TangentOracle:
         self.L.orc_tl_loss(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters, len(dts),
                            c(dts), c(X0), c(dX0), c(A0), c(dA0), len(g), ip(g), obs_every, c(obs),
                            int(freeze_limiter), dp(out), dp(dout))
+        return out[0], dout[0]
+
+    def chorin_loss(self, mesh, face_bc, p, X0, dX0, A0, dA0, dts, gauges, obs_every, obs, order=1,
+                    use_limiters=1, freeze_limiter=False, restart=30, max_it=3000, rtol=1e-14, atol=1e-30):
+        """(loss, directional derivative) over Chorin steps with frozen dts; the pressure's
+        tangent is that of the exact solve."""
+        L = self.L
+        L.orc_tl_chorin_loss.argtypes = [ctypes.POINTER(zdm._ZdmMesh), _ip, _dp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _ip, ctypes.c_int,
+                                         _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, _dp, _dp]
+        g = np.ascontiguousarray(gauges, dtype=np.int32)
+        out, dout = np.zeros(1), np.zeros(1)
+        c = lambda a: dp(np.ascontiguousarray(a, dtype=np.float64)) if a is not None else None
+        L.orc_tl_chorin_loss(mesh.handle, ip(face_bc), dp(p), order, use_limiters, len(dts), c(dts), c(X0), c(dX0),
+                             c(A0), c(dA0), len(g), ip(g), obs_every, c(obs), int(freeze_limiter), restart,
+                             max_it, rtol, atol, dp(out), dp(dout))
         return out[0], dout[0]
 
 
