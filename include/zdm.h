@@ -205,6 +205,17 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
  * counterpart in the reference, which has no differentiation path. */
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
                       const int* gauges, const double* obs, int exact_limiter);
+/* The same calibration problem over ChorinVAMSolver steps (VAM models; ChorinVAM.hpp:102-125:
+ * predictor, frozen derivative aux, pressure solve, corrector, constraints) with the Krylov
+ * settings of zdm_chorin_setup.  The gradient transposes the split with the pressure taken as
+ * the exact solution of its linear system: mu = A^-T (adjoint of P) by GMRES + point-block
+ * Jacobi on the transposed Pmat, then the residual's derivative with respect to the
+ * predicted state.  Layouts as above except the tapes (-1, -2): every layout re-runs each
+ * step once in the reverse sweep.  No counterpart in the reference. */
+int zdm_adjoint_setup_chorin(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
+                             const int* gauges, const double* obs, int exact_limiter);
+/* GMRES iterations of the transposed pressure solves since zdm_adjoint_setup[_chorin]. */
+int zdm_adjoint_krylov_its(zdm_solver* s, long long* its);
 /* Launch shape of the transport kernels (k_prep, k_flux, k_post): workgroup size 64, 128 or
  * 256 and 1..8 cells per thread; ensembles keep whole workgroups per member.  Results are
  * bitwise the same for every shape.  Invalidates a step captured by zdm_step_graph.  No

@@ -296,7 +296,8 @@ SOLVER_SYMBOLS = [
     "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres", "zdm_batch_create", "zdm_batch_time",
     "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
     "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout", "zdm_troubled_cells",
-    "zdm_set_launch", "zdm_get_launch", "zdm_autotune",
+    "zdm_set_launch", "zdm_get_launch", "zdm_autotune", "zdm_adjoint_setup_chorin",
+    "zdm_adjoint_krylov_its",
 ]
 
 
@@ -347,6 +348,8 @@ def model_lib(model):
     lib.zdm_step_graph.argtypes = [V, ctypes.c_int]
     lib.zdm_adjoint_setup.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       _c_int_p, _c_dbl_p, ctypes.c_int]
+    lib.zdm_adjoint_setup_chorin.argtypes = lib.zdm_adjoint_setup.argtypes
+    lib.zdm_adjoint_krylov_its.argtypes = [V, ctypes.POINTER(ctypes.c_longlong)]
     lib.zdm_adjoint_forward.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     lib.zdm_adjoint_gradient.argtypes = [V, _c_dbl_p, _c_dbl_p]
     lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
@@ -550,8 +553,10 @@ class Solver:
 
     # ---- discrete adjoint (calibration gradients)
     def adjoint_setup(self, nsteps, gauges=(), obs_every=0, obs=None, checkpoint_every=0,
-                      exact_limiter=False):
+                      exact_limiter=False, chorin=False):
         """Loss = sum over observation steps k*obs_every and gauges of (h - obs)^2.
+        chorin=True (VAM models): the steps are Chorin pressure-split steps with the Krylov
+        settings of chorin_setup; no tape layouts.
         exact_limiter=False holds each face's limiter value fixed (frozen-limiter
         linearisation); True differentiates it through its active branch.
         checkpoint_every: -2 / -1 tape with / without face states, 1 every state, k > 1 a
@@ -563,9 +568,15 @@ class Solver:
         if o.size != nobs * len(g):
             raise ZdmError("adjoint_setup: obs must have nsteps // obs_every rows of len(gauges)")
         self._adj = (int(nsteps), g, int(obs_every), o.reshape(nobs, len(g)))
-        self._check(self.lib.zdm_adjoint_setup(self._h, int(nsteps), int(checkpoint_every),
-                                               int(obs_every), len(g), g.ctypes.data_as(_c_int_p),
-                                               _dp(o) if o.size else None, int(bool(exact_limiter))))
+        fn = self.lib.zdm_adjoint_setup_chorin if chorin else self.lib.zdm_adjoint_setup
+        self._check(fn(self._h, int(nsteps), int(checkpoint_every), int(obs_every), len(g),
+                       g.ctypes.data_as(_c_int_p), _dp(o) if o.size else None, int(bool(exact_limiter))))
+
+    def adjoint_krylov_its(self):
+        """GMRES iterations of the transposed pressure solves since adjoint_setup."""
+        v = ctypes.c_longlong()
+        self._check(self.lib.zdm_adjoint_krylov_its(self._h, ctypes.byref(v)))
+        return v.value
 
     AUTOTUNE_CANDIDATES = [(256, 1), (128, 1), (64, 1), (256, 2), (128, 2), (64, 2), (256, 4), (128, 4),
                            (64, 4)]

@@ -943,10 +943,11 @@ __global__ __launch_bounds__(BLOCK) void k_post_mood(DevMesh m, Params P, Ctrl* 
     if (threadIdx.x == 0) partials[L.lb] = ratio;
 }
 
-// k_post<POST> (POST 1: constraints; POST 2: constraints, source, constraints) of a step whose
-// intermediate states are kept: reads the final stage's output X1, writes the source's solution
-// X3 and the aux A3 it saw (POST 2), the end-of-step state Xout and its aux A, and the CFL
-// partial minima; the same operations as k_post.  dt_log (may be null) receives the step's dt.
+// k_post<POST> (POST 1: constraints; POST 2: constraints, source, constraints; POST 5: source,
+// constraints, the Chorin predictor's) of a step whose intermediate states are kept: reads the
+// final stage's output X1, writes the source's solution X3 and the aux A3 it saw (POST 2, 5),
+// the end-of-step state Xout and its aux A, and the CFL partial minima; the same operations as
+// k_post.  dt_log (may be null) receives the step's dt.
 template <int POST, bool MC>
 __global__ __launch_bounds__(BLOCK) void k_post_record(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                        const T* __restrict__ X1, T* __restrict__ Xout, T* A,
@@ -962,8 +963,8 @@ __global__ __launch_bounds__(BLOCK) void k_post_record(DevMesh m, Params P, cons
         T q[NQ], a[NA];
         load_soa<NQ>(X1, ld, c, q);
         load_soa<NA>(A, ld, c, a);
-        update_state_cell(P, q, a);
-        if constexpr (POST == 2) {
+        if constexpr (POST != 5) update_state_cell(P, q, a);
+        if constexpr (POST == 2 || POST == 5) {
             store_soa<NA>(A3, ld, c, a);
             newton_cell(P, ctrl->dt, q, a);
             store_soa<NQ>(X3, ld, c, q);
@@ -1071,6 +1072,7 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
 #include "adjoint.inc"
 #ifdef ZOOMY_MODEL_IS_VAM
 #include "chorin.inc"
+#include "chorin_adjoint.inc"
 #endif
 struct ChorinHost;
 struct AdjointHost;
@@ -2169,8 +2171,22 @@ extern "C" {
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
                       const int* gauges, const double* obs, int exact_limiter) {
     if (!s) return fail("null solver");
-    if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs)) return -1;
+    if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 0)) return -1;
     s->adj->freeze_limiter = exact_limiter ? 0 : 1;
+    return 0;
+}
+
+int zdm_adjoint_setup_chorin(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
+                             const int* gauges, const double* obs, int exact_limiter) {
+    if (!s) return fail("null solver");
+    if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 1)) return -1;
+    s->adj->freeze_limiter = exact_limiter ? 0 : 1;
+    return 0;
+}
+
+int zdm_adjoint_krylov_its(zdm_solver* s, long long* its) {
+    if (!s || !s->adj || !its) return fail("zdm_adjoint_krylov_its: bad arguments");
+    *its = s->adj->its_t;
     return 0;
 }
 
