@@ -203,6 +203,82 @@ def run_c3(zdm, cases, args):
     print(json.dumps(out))
 
 
+def run_c3adj(zdm, cases, args):
+    """Calibration gradient through the Chorin split (C3 mesh and solver): recording forward +
+    reverse sweep (re-run of each step, transposed pressure solve, transposed kernels) against
+    the plain forward run of the same steps."""
+    n = args.grid if args.grid != 1000 else 707
+    mesh = zdm.Mesh.tri_grid(n, n, jitter=0.15, seed=3, renumber=args.order)
+    Xt = cases.c3_state(mesh)
+    Xg = cases.c3_state(mesh, amp=0.18)
+    krylov = dict(restart=30, max_it=args.max_it, rtol=1e-10, atol=1e-12, op_mode=args.op_mode)
+    s = zdm.Solver("vam", mesh, order=1, cfl=0.45, t_end=1e9, source=True)
+    s.chorin_setup(**krylov)
+    T, obs_every = args.steps, 5
+    rng = np.random.default_rng(3)
+    gauges = rng.choice(mesh.ncells, 32, replace=False).astype(np.int32)
+    # synthetic observations: h at the gauges every 5 steps of the hidden-truth run
+    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, chorin=True)
+    s.adjoint_forward(Xt, None)
+    obs = s.adjoint_residuals().copy()
+    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs, checkpoint_every=args.checkpoint_every,
+                    chorin=True)
+    ck, _ = s.adjoint_layout()
+    s.upload(Xg, None)
+    s.initialize()
+    s.chorin_step(min(args.warmup, 3))
+    s.upload(Xg, None)
+    s.initialize()
+    s.synchronize()
+    t0 = time.perf_counter()
+    steps, its, _ = s.chorin_step(T)
+    s.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = s.adjoint_forward(Xg, None)
+    t_fwd = time.perf_counter() - t0
+    its_t0 = s.adjoint_krylov_its()
+    t0 = time.perf_counter()
+    gX, _ = s.adjoint_gradient()
+    t_rev = time.perf_counter() - t0
+    its_t = s.adjoint_krylov_its() - its_t0
+    if not np.all(np.isfinite(gX)):
+        raise RuntimeError("non-finite gradient")
+    ncells = mesh.ncells
+    out = {
+        "metric": "adjoint/forward ratio (VAM Chorin calibration, FP64)",
+        "value": round((t_fwd + t_rev) / t_plain, 3),
+        "unit": "x forward time",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": min(args.warmup, 3),
+        "ms_per_step": round(t_plain / T * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "VAM level 1 Chorin split, %dx%d grid jittered +-15%% (%d cells), %d steps, "
+                        "32 gauges every %d steps, gradient wrt the initial state; GMRES(30) + PBJ "
+                        "rtol 1e-10 forward and transposed" % (n, n, ncells, T, obs_every),
+            "cells": ncells,
+            "forward_s": round(t_plain, 4),
+            "forward_recording_s": round(t_fwd, 4),
+            "reverse_s": round(t_rev, 4),
+            "gmres_its_forward": its,
+            "gmres_its_transposed": its_t,
+            "loss": loss,
+            "grad_norm_bed": float(np.linalg.norm(gX[:, 0])),
+            "grad_norm_h": float(np.linalg.norm(gX[:, 1])),
+            "checkpoint_every": ck,
+        },
+        "roofline": None,
+    }
+    s.close()
+    print(json.dumps(out))
+
+
 def c4_members(zdm, cases, count):
     meshes, states = [], []
     for i in range(count):
@@ -387,9 +463,10 @@ def run_c5(zdm, cases, args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c3adj"],
                     help="c2: SME level 2 transport + source (headline); c3: VAM Chorin split; "
-                         "c4: batched vs serial ensemble; c5: adjoint calibration gradient")
+                         "c4: batched vs serial ensemble; c5: adjoint calibration gradient; "
+                         "c3adj: adjoint through the Chorin split")
     ap.add_argument("--members", type=int, default=256, help="c4 ensemble size")
     ap.add_argument("--op-mode", type=int, default=0, choices=[0, 1],
                     help="c3 pressure operator: 0 matrix-free residual, 1 assembled Pmat")
@@ -400,7 +477,7 @@ def main():
                          "the richest that fits")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: c2 200, c3 10, c4 300, c5 500)")
+                    help="timed steps (default: c2 200, c3 10, c4 300, c5 500, c3adj 10)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--grid", type=int, default=1000)
     ap.add_argument("--order", default="rcm", choices=["hilbert", "rcm", "none"],
@@ -413,13 +490,15 @@ def main():
         sys.exit(2)
 
     if args.steps is None:
-        args.steps = {"c2": 200, "c3": 10, "c4": 300, "c5": 500}[args.workload]
+        args.steps = {"c2": 200, "c3": 10, "c4": 300, "c5": 500, "c3adj": 10}[args.workload]
     zdm = load_package()
     import zoomy_dmplex_amd.cases as cases
     if args.workload == "c5":
         return run_c5(zdm, cases, args)
     if args.workload == "c3":
         return run_c3(zdm, cases, args)
+    if args.workload == "c3adj":
+        return run_c3adj(zdm, cases, args)
     if args.workload == "c4":
         return run_c4(zdm, cases, args)
 
