@@ -468,8 +468,9 @@ def main():
                          "c4: batched vs serial ensemble; c5: adjoint calibration gradient; "
                          "c3adj: adjoint through the Chorin split")
     ap.add_argument("--members", type=int, default=256, help="c4 ensemble size")
-    ap.add_argument("--op-mode", type=int, default=0, choices=[0, 1],
-                    help="c3 pressure operator: 0 matrix-free residual, 1 assembled Pmat")
+    ap.add_argument("--op-mode", type=int, default=1, choices=[0, 1],
+                    help="c3 pressure operator: 1 the device-assembled Pmat (sliced ELL SpMV, "
+                         "default), 0 the matrix-free residual")
     ap.add_argument("--max-it", type=int, default=400, help="c3 GMRES iteration cap")
     ap.add_argument("--checkpoint-every", type=int, default=0,
                     help="c5 trajectory layout: -2/-1 tape with/without face states, 1 every "
