@@ -30,7 +30,8 @@ typedef struct zdm_options {
     int ssp_stages;     /* s of the s-stage SSP-RK2 (TSSSP "rks2", SolverStrategies.hpp:33-35) */
     int use_limiters;   /* limiter != "none" -> Vacuum + TVD (MUSCLSolver.hpp:40-43) */
     int gradient;       /* 0 = Green-Gauss (reference, ModularSolver.hpp:62), 1 = least squares */
-    int source;         /* time_integration == "splitting": split implicit source (MUSCLSolver.hpp:179) */
+    int source;         /* the model's source: split implicit step (splitting, MUSCLSolver.hpp:179)
+                           or the implicit part of the IMEX step */
     double cfl;         /* solver.cfl */
     double t_end;       /* solver.t_end */
     double min_dt;      /* solver.min_dt (first step only, MUSCLSolver.hpp:103) */
@@ -42,6 +43,10 @@ typedef struct zdm_options {
                            h < -1e-10, MUSCLSolver.hpp:130-171), 2 "zhang_shu" (order 2: eta-WB +
                            Zhang-Shu theta, CFL capped at 1/6, MUSCLSolver.hpp:31-38) */
     double wet_dry_eps; /* solver.wet_dry_eps (<= 0: the default 1e-2) */
+    int time_integration; /* solver.time_integration: 0 "splitting" (SSP-RK / Euler transport, then
+                           the split implicit source, SplittingStrategy), 1 "imex" (explicit
+                           transport + implicit source in one additive Runge-Kutta step,
+                           IMEXStrategy, SolverStrategies.hpp:62-84) */
 } zdm_options;
 
 /* Model plug-in description (Model.H:31-44). */

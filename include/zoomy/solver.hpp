@@ -12,7 +12,7 @@
 //   ChorinVAMSolver ChorinVAM.hpp:86-129       (Run; KSP options from -ksp_* arguments)
 //   TransportStep   TransportStep.hpp:43-565   (FormRHS, UpdateState, setters)
 //   SourceStep      SourceStep.hpp:56-122      (Solve)
-//   SolverStrategy  SolverStrategies.hpp:8-57  (SplittingStrategy)
+//   SolverStrategy  SolverStrategies.hpp:8-84  (SplittingStrategy, IMEXStrategy)
 //   GradientCalculator / Reconstructor / Limiter: the configuration they carry
 // Error handling mirrors PetscErrorCode: every call returns 0 on success and a non-zero
 // code otherwise, with the message in LastError().  The model is the one the linked
@@ -231,6 +231,18 @@ public:
     const char* Name() const override { return "splitting"; }
     ErrorCode SetupTS(ModularSolver*, zdm_options& o) override {
         o.source = 1;
+        o.time_integration = 0;
+        return 0;
+    }
+};
+// explicit transport + implicit source in one additive Runge-Kutta step (TSARKIMEX "2e",
+// SolverStrategies.hpp:62-84)
+class IMEXStrategy : public SolverStrategy {
+public:
+    const char* Name() const override { return "imex"; }
+    ErrorCode SetupTS(ModularSolver*, zdm_options& o) override {
+        o.source = 1;
+        o.time_integration = 1;
         return 0;
     }
 };
@@ -465,9 +477,11 @@ public:
         if (!strategy) {
             if (ss.time_integration == "splitting")
                 strategy = std::make_shared<SplittingStrategy>();
+            else if (ss.time_integration == "imex")
+                strategy = std::make_shared<IMEXStrategy>();
             else
                 return Fail("time_integration '" + ss.time_integration + "' is not provided by this build "
-                            "(splitting only)");
+                            "(splitting, imex)");
         }
         int positivity = 0;
         if (ss.positivity == "mood")

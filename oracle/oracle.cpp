@@ -673,6 +673,88 @@ void ssp_or_euler(int order, int s, size_t n, T t0, T dt, T* X, RHS&& rhs) {
     for (size_t k = 0; k < n; ++k) X[k] = al * w0[k] + be * F[k] + ga * X[k];
 }
 
+// IMEX step (IMEXStrategy, SolverStrategies.hpp:62-84: TSARKIMEX "2e", RHSFunction = FormRHS,
+// IFunction = Xdot - source): the additive Runge-Kutta method ARK2 of Giraldo, Kelly and
+// Constantinescu, SIAM J. Sci. Comput. 35(5), 2013, B1162-B1194 -- explicit first stage,
+// L-stable SDIRK implicit part with diagonal g = 1 - 1/sqrt(2), weights b = (d, d, g),
+// d = 1/(2 sqrt(2)), explicit a32 = (3 + 2 sqrt(2))/6, c = (0, 2g, 1).  PETSc itself is not
+// available here: which member of the family its "2e" registers is unpinned.
+//   Y1 = X;  Yi = X + dt sum_{j<i} (a_ij G_j + at_ij S_j) + dt g S(Yi);  X+ = X + dt sum b_i (G_i + S_i)
+// rhs(t, U, F) = G; src(U, S) = S; solve(dg, Z, Y): Y - dg S(Y) = Z.  S_i = (Y_i - Z_i)/(dt g).
+namespace ark2o {
+const double R2 = 1.41421356237309504880;
+const double G = 1.0 - 1.0 / R2, D = 1.0 / (2.0 * R2);
+const double A21 = 2.0 * G, A32 = (3.0 + 2.0 * R2) / 6.0, A31 = 1.0 - A32;
+const double AT21 = G, AT31 = D, AT32 = D;
+const double B1 = D, B2 = D, B3 = G, C2 = 2.0 * G;
+}  // namespace ark2o
+
+template <typename RHS, typename SRC, typename SOLVE>
+void ark2_step(size_t n, T t0, T dt, T* X, RHS&& rhs, SRC&& src, SOLVE&& solve) {
+    using namespace ark2o;
+    std::vector<T> G1(n), G2(n), G3(n), S1(n), S2(n), S3(n), Y(n), Z(n);
+    const T dg = dt * G;
+    rhs(t0, X, G1.data());
+    src(X, S1.data());
+    for (size_t k = 0; k < n; ++k) Z[k] = X[k] + dt * (A21 * G1[k] + AT21 * S1[k]);
+    solve(dg, Z.data(), Y.data());
+    for (size_t k = 0; k < n; ++k) S2[k] = (Y[k] - Z[k]) / dg;
+    rhs(t0 + C2 * dt, Y.data(), G2.data());
+    for (size_t k = 0; k < n; ++k) Z[k] = X[k] + dt * (A31 * G1[k] + A32 * G2[k] + AT31 * S1[k] + AT32 * S2[k]);
+    solve(dg, Z.data(), Y.data());
+    for (size_t k = 0; k < n; ++k) S3[k] = (Y[k] - Z[k]) / dg;
+    rhs(t0 + dt, Y.data(), G3.data());
+    for (size_t k = 0; k < n; ++k) X[k] = X[k] + dt * (B1 * (G1[k] + S1[k]) + B2 * (G2[k] + S2[k]) + B3 * (G3[k] + S3[k]));
+}
+
+// the IFunction's source (AddImplicitSourceToResidual, ModularSolver.hpp:82-124): per cell on
+// the clamped state with refreshed aux (UpdateState of a local copy), zero where it is dry
+// (h < 1e-6); source(Q, Qaux, p, 0, 0, 0)
+void imex_source(int nc, const T* p, const T* U, const T* A, T* S) {
+    const T X0[3] = {0, 0, 0};
+    for (int c = 0; c < nc; ++c) {
+        T q[NQ], a[NA];
+        for (int i = 0; i < NQ; ++i) q[i] = U[(size_t)c * NQ + i];
+        for (int i = 0; i < NA; ++i) a[i] = A[(size_t)c * NA + i];
+        update_state(1, p, q, a);
+        T* out = S + (size_t)c * NQ;
+        if (q[1] < 1e-6) {
+            for (int i = 0; i < NQ; ++i) out[i] = 0;
+            continue;
+        }
+        auto sv = M::source(q, a, p, (T)0, (T)0, X0);
+        for (int i = 0; i < NQ; ++i) out[i] = sv[i];
+    }
+}
+
+int g_time_integration = 0;   // 0 splitting, 1 imex
+
+void imex_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, int source, T t0, T dt, T* X,
+               const T* A, Work& w) {
+    const int nc = m->ncells;
+    const size_t n = (size_t)nc * NQ;
+    ark2_step(
+        n, t0, dt, X, [&](T t, const T* U, T* F) { form_rhs(m, face_bc, p, o, t, U, A, F, w); },
+        [&](const T* U, T* S) {
+            if (source)
+                imex_source(nc, p, U, A, S);
+            else
+                std::fill(S, S + n, (T)0);
+        },
+        [&](T dg, const T* Z, T* Y) {
+            // per cell: Y = Z, then the SourceStep Newton unless the clamped Z is dry
+            for (size_t k = 0; k < n; ++k) Y[k] = Z[k];
+            if (!source) return;
+            for (int c = 0; c < nc; ++c) {
+                T qc[NQ], ac[NA];
+                for (int i = 0; i < NQ; ++i) qc[i] = Z[(size_t)c * NQ + i];
+                for (int i = 0; i < NA; ++i) ac[i] = A[(size_t)c * NA + i];
+                update_state(1, p, qc, ac);
+                if (qc[1] >= 1e-6) solve_local_cell(dg, Y + (size_t)c * NQ, A + (size_t)c * NA, p);
+            }
+        });
+}
+
 void ts_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T t0, T dt, T* X,
              const T* A, Work& w) {
     const size_t n = (size_t)m->ncells * NQ;
@@ -843,6 +925,23 @@ int orc_ts_step(const zdm_mesh* m, const int* face_bc, const double* p, int orde
 
 // solver.positivity / wet_dry_eps for the following calls (0 none, 1 mood, 2 zhang_shu); also
 // resets the MOOD override count
+// solver.time_integration of orc_run: 0 "splitting", 1 "imex"
+int orc_set_time_integration(int ti) {
+    g_time_integration = ti;
+    return 0;
+}
+
+// ARK2 on the scalar test equation y' = le y + li y (le explicit, li implicit): n steps of dt
+// from y0, through the same stage code as the mesh step
+double orc_ark2_linear(double le, double li, double dt, int n, double y0) {
+    T y = y0;
+    for (int k = 0; k < n; ++k)
+        ark2_step(
+            1, 0.0, dt, &y, [&](T, const T* U, T* F) { F[0] = le * U[0]; }, [&](const T* U, T* S) { S[0] = li * U[0]; },
+            [&](T dg, const T* Z, T* Y) { Y[0] = Z[0] / (1.0 - dg * li); });
+    return y;
+}
+
 int orc_set_positivity(int positivity, double wet_dry_eps) {
     g_positivity = positivity;
     g_wet_eps = wet_dry_eps;
@@ -889,7 +988,10 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
         double t0 = t, dt0 = dt;
         if (t0 + dt0 > t_end) dt0 = t_end - t0;
         if (g_positivity == 1) Xs.assign(X, X + n);
-        ts_step(m, face_bc, p, o, t0, dt0, X, A, w);
+        if (g_time_integration == 1)
+            imex_step(m, face_bc, p, o, 1, t0, dt0, X, A, w);
+        else
+            ts_step(m, face_bc, p, o, t0, dt0, X, A, w);
         enforce(m->ncells, p, X, A);
         if (g_positivity == 1) {
             mask.assign(m->ncells, 0);
@@ -913,8 +1015,10 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
                 g_last_mask = mask;
             }
         }
-        source_solve(m->ncells, p, dt0, X, A);
-        enforce(m->ncells, p, X, A);
+        if (g_time_integration == 0) {
+            source_solve(m->ncells, p, dt0, X, A);
+            enforce(m->ncells, p, X, A);
+        }
         ++steps;
         t = t0 + dt0;
         dt = compute_time_step(m, p, X, A, cfl, nullptr);

@@ -152,11 +152,12 @@ class Oracle:
 
     def run(self, mesh, face_bc, p, X, A, order=2, ssp_stages=2, use_limiters=1, gradient=0,
             cfl=0.45, t_end=1e9, min_dt=1e-12, max_steps=10, time=0.0, dt=0.0, first=True,
-            well_balanced=0, positivity="none", wet_dry_eps=1e-2):
+            well_balanced=0, positivity="none", wet_dry_eps=1e-2, time_integration="splitting"):
         """MUSCLSolver loop; returns (steps, time, next dt).  self.troubled = MOOD overrides."""
         t = np.array([time])
         d = np.array([dt])
         self.L.orc_set_positivity(self.POSITIVITY[positivity], float(wet_dry_eps))
+        self.L.orc_set_time_integration({"splitting": 0, "imex": 1}[time_integration])
         try:
             n = self.L.orc_run(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters,
                                gradient, int(well_balanced), cfl, t_end, min_dt, max_steps, dp(X),
@@ -167,6 +168,7 @@ class Oracle:
             self.last_troubled = mask.astype(bool)
         finally:
             self.L.orc_set_positivity(0, 1e-2)
+            self.L.orc_set_time_integration(0)
         return n, t[0], d[0]
 
 
@@ -225,6 +227,13 @@ class Oracle:
                                   dp(A), dp(P), dp(t), ctypes.byref(its), ctypes.byref(reason))
         return n, t[0], its.value, reason.value
 
+
+    def ark2_linear(self, le, li, dt, n, y0=1.0):
+        """The oracle's ARK2 step on y' = le y (explicit) + li y (implicit), n steps."""
+        self.L.orc_ark2_linear.restype = ctypes.c_double
+        self.L.orc_ark2_linear.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_double]
+        return self.L.orc_ark2_linear(le, li, dt, n, y0)
 
     def chorin_loss(self, mesh, face_bc, p, X0, A0, dts, gauges, obs_every, obs, order=1, use_limiters=1,
                     restart=30, max_it=3000, rtol=1e-14, atol=1e-30):

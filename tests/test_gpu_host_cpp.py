@@ -30,8 +30,9 @@ def run_cli(model, settings, tmp_path, *extra):
     return np.load(out), r.stdout
 
 
-@pytest.mark.parametrize("positivity", ["none", "mood", "zhang_shu"])
-def test_cli_muscl_equals_python_solver(tmp_path, positivity):
+@pytest.mark.parametrize("positivity,ti", [("none", "splitting"), ("mood", "splitting"), ("zhang_shu", "splitting"),
+                                           ("none", "imex"), ("mood", "imex")])
+def test_cli_muscl_equals_python_solver(tmp_path, positivity, ti):
     mesh = zdm.Mesh.read_msh(MESH)
     xy = mesh.cell_centroid[:, :2]
     Xs = np.zeros((mesh.ncells, 6))
@@ -41,7 +42,7 @@ def test_cli_muscl_equals_python_solver(tmp_path, positivity):
     np.save(tmp_path / "ic.npy", input_order_state(mesh, Xs))
     settings = {"name": "cli", "io": {"mesh_path": MESH, "initial_condition_file": str(tmp_path / "ic.npy")},
                 "solver": {"t_end": 1e9, "cfl": 0.45, "reconstruction_order": 2, "ssp_stages": 2,
-                           "limiter": "tvd", "positivity": positivity}}
+                           "limiter": "tvd", "positivity": positivity, "time_integration": ti}}
     Xc, log = run_cli("sme1", settings, tmp_path, "-max_steps", "25", "-no_output")
     assert "Finished" in log
     s = zdm.Solver("sme1", mesh, settings=zdm.Settings(settings))
@@ -80,11 +81,11 @@ def test_cli_refuses_what_it_does_not_provide(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     settings["solver"]["method"] = "muscl"
-    settings["solver"]["time_integration"] = "imex"
+    settings["solver"]["time_integration"] = "implicit"
     sp.write_text(json.dumps(settings))
     r = subprocess.run([os.path.join(PKG_DIR, "zdm_cli_sme1"), "-settings", str(sp)],
                        capture_output=True, text=True, timeout=120)
-    assert r.returncode != 0 and "imex" in r.stderr
+    assert r.returncode != 0 and "implicit" in r.stderr
 
 
 def read_vtu(path):

@@ -277,9 +277,10 @@ class _Options(ctypes.Structure):
                 ("source", ctypes.c_int), ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
                 ("min_dt", ctypes.c_double), ("block_size", ctypes.c_int),
                 ("well_balanced", ctypes.c_int), ("positivity", ctypes.c_int),
-                ("wet_dry_eps", ctypes.c_double)]
+                ("wet_dry_eps", ctypes.c_double), ("time_integration", ctypes.c_int)]
 
 POSITIVITY = {"none": 0, "mood": 1, "zhang_shu": 2}
+TIME_INTEGRATION = {"splitting": 0, "imex": 1}
 
 
 _model_libs = {}
@@ -407,7 +408,7 @@ class Solver:
     def __init__(self, model, mesh, settings=None, params=None, face_bc=None, source=True,
                  order=None, ssp_stages=None, cfl=None, t_end=None, gradient=None,
                  use_limiters=None, well_balanced=None, cell_offsets=None, positivity=None,
-                 wet_dry_eps=None):
+                 wet_dry_eps=None, time_integration=None):
         self.model = model
         self.info = ModelInfo(model)
         self.lib = model_lib(model)
@@ -439,7 +440,12 @@ class Solver:
         o.use_limiters = int(use_limiters if use_limiters is not None else sv["limiter"] != "none")
         g = gradient if gradient is not None else sv.get("gradient", "green_gauss")
         o.gradient = 1 if g in ("lsq", "least_squares", 1) else 0
-        o.source = int(source and sv["time_integration"] == "splitting")
+        ti = time_integration if time_integration is not None else sv["time_integration"]
+        if ti not in TIME_INTEGRATION:
+            raise ZdmError("time_integration %r is not provided by this build (%s)"
+                           % (ti, ", ".join(TIME_INTEGRATION)))
+        o.time_integration = TIME_INTEGRATION[ti]
+        o.source = int(bool(source))
         o.cfl = float(cfl if cfl is not None else sv["cfl"])
         o.t_end = float(t_end if t_end is not None else sv["t_end"])
         o.min_dt = float(sv["min_dt"])

@@ -1068,6 +1068,7 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
 }
 
 #include "krylov.inc"
+#include "imex.inc"
 #include "binomial.h"
 #include "adjoint.inc"
 #ifdef ZOOMY_MODEL_IS_VAM
@@ -1097,6 +1098,7 @@ struct zdm_solver {
     DevMesh dm;
     T *X, *W, *F, *A, *As, *Atmp, *QF, *partials, *stage_aos;
     T* Xs;                            // MOOD: the state at the start of the step
+    T* Im;                            // IMEX: stage right-hand sides G1..3, sources S1..3, stage state
     double cfl_in;                    // solver.cfl as given (opt.cfl: capped for zhang_shu)
     Ctrl *ctrl, *ctrl_rhs;
     bool timing;
@@ -1244,7 +1246,7 @@ void flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* In, T* 
 // post-stage with the MOOD override (positivity "mood"): reads the saved state s->Xs
 void post_mood(zdm_solver* s) {
     dim3 g(s->nblocks), b(s->bs);
-    const bool src = s->opt.source;
+    const bool src = s->opt.source && s->opt.time_integration == 0;   // IMEX: no split source
 #define ZDM_MOOD1(MF, MC)                                                                                            \
     if (src)                                                                                                         \
         k_post_mood<MF, 1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->Xs, s->As, s->partials); \
@@ -1387,6 +1389,10 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     if (opts->positivity != 0 && opts->well_balanced)
         return fail("positivity and the well_balanced extension cannot be combined");
     if (opts->positivity != 0 && NQ < 2) return fail("positivity needs the [b, h, ...] state layout");
+    if (opts->time_integration < 0 || opts->time_integration > 1)
+        return fail("time_integration must be 0 (splitting) or 1 (imex)");
+    if (opts->time_integration == 1 && nmem > 0) return fail("time_integration imex: not available for an ensemble");
+    if (opts->time_integration == 1 && NQ < 2) return fail("time_integration imex needs the [b, h, ...] state layout");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
 
@@ -1508,6 +1514,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     size_t oX = take(sQ), oW = take(sQ), oF = take(sQ), oA = take(sA), oAs = take(sA), oAt = take(sA);
     size_t oQF = take(sizeof(T) * (size_t)MFa * NQ * ld);
     size_t oXs = take(s->opt.positivity == 1 ? sQ : 8);   // MOOD: the saved start-of-step state
+    size_t oIm = take(s->opt.time_integration == 1 ? 7 * sQ : 8);   // IMEX: G1..3, S1..3, stage state
     size_t oPart = take(sizeof(T) * (size_t)s->maxblocks);
     size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
     size_t oCtrl = take(sizeof(Ctrl) * s->nctrl), oCtrlR = take(sizeof(Ctrl) * s->nctrl);
@@ -1534,6 +1541,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     char* a = s->arena;
     s->X = (T*)(a + oX);
     s->Xs = (T*)(a + oXs);
+    s->Im = (T*)(a + oIm);
     s->W = (T*)(a + oW);
     s->F = (T*)(a + oF);
     s->A = (T*)(a + oA);
@@ -1756,8 +1764,51 @@ static int resolve_events(zdm_solver* s) {
     return 0;
 }
 
+// the launches of one IMEX step (imex.inc): three transport right-hand sides, the per-cell
+// implicit stages, the combination; then constraints [with the MOOD override] and the CFL dt as
+// after any step (no split source)
+static void enqueue_imex_step(zdm_solver* s) {
+    const size_t nQ = (size_t)NQ * s->ld;
+    T* G = s->Im;
+    T* S = s->Im + 3 * nQ;
+    T* Y = s->Im + 6 * nQ;
+    const int src = s->opt.source;
+    dim3 g(s->ncb), b(BLOCK);
+    if (s->opt.positivity == 1)
+        (void)hipMemcpyAsync(s->Xs, s->X, sizeof(T) * nQ, hipMemcpyDeviceToDevice, s->stream);
+    record(s, -1);
+    k_imex_times<<<1, 1, 0, s->stream>>>(s->ctrl);
+    prep(s, s->ctrl, s->X, s->A);
+    record(s, 0);
+    flux(s, s->ctrl, MODE_RHS, 0, s->X, G, s->A);
+    record(s, 1);
+    k_imex_s1<<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, S, src);
+    k_imex_stage<2><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, G, nullptr, S, nullptr, Y, S + nQ, src);
+    prep(s, s->ctrl, Y, s->A);
+    record(s, 0);
+    flux(s, s->ctrl, MODE_RHS, 1, Y, G + nQ, s->A);
+    record(s, 1);
+    k_imex_stage<3><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, G, G + nQ, S, S + nQ, Y, S + 2 * nQ,
+                                             src);
+    prep(s, s->ctrl, Y, s->A);
+    record(s, 0);
+    flux(s, s->ctrl, MODE_RHS, 2, Y, G + 2 * nQ, s->A);
+    record(s, 1);
+    k_imex_final<<<g, b, 0, s->stream>>>(s->dm, s->ctrl, s->X, G, S, nQ);
+    if (s->opt.positivity == 1)
+        post_mood(s);
+    else
+        post(s, 1);
+    record(s, 2);
+    finalize(s, 2);
+}
+
 // the launches of one MUSCLSolver time step
 static void enqueue_step(zdm_solver* s) {
+    if (s->opt.time_integration == 1) {
+        enqueue_imex_step(s);
+        return;
+    }
     const int order = s->opt.order, st = s->opt.ssp_stages;
     const bool mood = s->opt.positivity == 1;
     if (mood) (void)hipMemcpyAsync(s->Xs, s->X, sizeof(T) * (size_t)NQ * s->ld, hipMemcpyDeviceToDevice, s->stream);
