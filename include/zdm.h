@@ -46,8 +46,20 @@ typedef struct zdm_options {
     int time_integration; /* solver.time_integration: 0 "splitting" (SSP-RK / Euler transport, then
                            the split implicit source, SplittingStrategy), 1 "imex" (explicit
                            transport + implicit source in one additive Runge-Kutta step,
-                           IMEXStrategy, SolverStrategies.hpp:62-84) */
+                           IMEXStrategy, SolverStrategies.hpp:62-84), 2 "implicit" (BDF2 on
+                           transport + source, Newton-Krylov, FullyImplicitStrategy,
+                           SolverStrategies.hpp:88-195) */
 } zdm_options;
+
+/* Newton-Krylov settings of time_integration "implicit" (the SNES / KSP of the TS). */
+typedef struct zdm_newton_options {
+    double snes_atol;   /* 1e-4 (MUSCLSolver.hpp:108-110) */
+    double snes_rtol;   /* 1e-3 */
+    int snes_max_it;    /* 20 */
+    double ksp_rtol;    /* PETSc default 1e-5 */
+    int ksp_restart;    /* GMRES restart, PETSc default 30 */
+    int ksp_max_it;     /* PETSc default 10000 */
+} zdm_newton_options;
 
 /* Model plug-in description (Model.H:31-44). */
 int zdm_model_info(int* n_q, int* n_aux, int* n_params, int* dim);
@@ -221,6 +233,12 @@ int zdm_adjoint_setup_chorin(zdm_solver* s, int nsteps, int checkpoint_every, in
                              const int* gauges, const double* obs, int exact_limiter);
 /* GMRES iterations of the transposed pressure solves since zdm_adjoint_setup[_chorin]. */
 int zdm_adjoint_krylov_its(zdm_solver* s, long long* its);
+/* time_integration "implicit": Newton / Krylov settings (NULL restores the defaults above)
+ * and the counters since creation (Newton and GMRES iterations, the last step's Newton
+ * iterations and SNESConvergedReason: 2 |G| < atol, 3 |G| < rtol |G0|, -5 max_it, -4 NaN). */
+int zdm_implicit_setup(zdm_solver* s, const zdm_newton_options* opts);
+int zdm_implicit_stats(zdm_solver* s, long long* newton_its, long long* ksp_its, int* last_newton,
+                       int* last_reason);
 /* Launch shape of the transport kernels (k_prep, k_flux, k_post): workgroup size 64, 128 or
  * 256 and 1..8 cells per thread; ensembles keep whole workgroups per member.  Results are
  * bitwise the same for every shape.  Invalidates a step captured by zdm_step_graph.  No

@@ -12,7 +12,7 @@
 //   ChorinVAMSolver ChorinVAM.hpp:86-129       (Run; KSP options from -ksp_* arguments)
 //   TransportStep   TransportStep.hpp:43-565   (FormRHS, UpdateState, setters)
 //   SourceStep      SourceStep.hpp:56-122      (Solve)
-//   SolverStrategy  SolverStrategies.hpp:8-84  (SplittingStrategy, IMEXStrategy)
+//   SolverStrategy  SolverStrategies.hpp:8-195 (SplittingStrategy, IMEXStrategy, FullyImplicitStrategy)
 //   GradientCalculator / Reconstructor / Limiter: the configuration they carry
 // Error handling mirrors PetscErrorCode: every call returns 0 on success and a non-zero
 // code otherwise, with the message in LastError().  The model is the one the linked
@@ -243,6 +243,17 @@ public:
     ErrorCode SetupTS(ModularSolver*, zdm_options& o) override {
         o.source = 1;
         o.time_integration = 1;
+        return 0;
+    }
+};
+// transport and source implicit together: TSBDF order 2, Newton-Krylov with a matrix-free
+// Jacobian (SolverStrategies.hpp:88-195)
+class FullyImplicitStrategy : public SolverStrategy {
+public:
+    const char* Name() const override { return "implicit"; }
+    ErrorCode SetupTS(ModularSolver*, zdm_options& o) override {
+        o.source = 1;
+        o.time_integration = 2;
         return 0;
     }
 };
@@ -479,9 +490,11 @@ public:
                 strategy = std::make_shared<SplittingStrategy>();
             else if (ss.time_integration == "imex")
                 strategy = std::make_shared<IMEXStrategy>();
+            else if (ss.time_integration == "implicit")
+                strategy = std::make_shared<FullyImplicitStrategy>();
             else
                 return Fail("time_integration '" + ss.time_integration + "' is not provided by this build "
-                            "(splitting, imex)");
+                            "(splitting, imex, implicit)");
         }
         int positivity = 0;
         if (ss.positivity == "mood")

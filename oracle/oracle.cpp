@@ -755,6 +755,92 @@ void imex_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o,
         });
 }
 
+#ifndef ORACLE_TANGENT
+// Fully implicit step (FullyImplicitStrategy, SolverStrategies.hpp:88-195: TSBDF order 2 on the
+// IFunction Xdot - source - FormRHS): the variable-step BDF2 equation
+//   G(X) = (c0 X + c1 X_n + c2 X_{n-1} - S(X)) - R(X) = 0
+// with c the derivative at t_{n+1} of the Lagrange interpolant through t_{n-1}, t_n, t_{n+1}
+// (BDF1, c = (1/h, -1/h, 0), for the first step), R = FormRHS at t_{n+1}, S as in the IMEX
+// step.  The oracle solves it to round-off by Newton with a dense central-difference
+// Jacobian: it checks the discrete equation, not a Krylov path to it.  PETSc's own Newton
+// stops at its SNES tolerances; that, and its TSBDF start-up, are parity unpinned.
+void bdf_coeffs(double h, double h1, bool have_prev, double* c) {
+    if (have_prev) {
+        c[0] = 1.0 / h + 1.0 / (h + h1);
+        c[1] = -(h + h1) / (h * h1);
+        c[2] = h / (h1 * (h + h1));
+    } else {
+        c[0] = 1.0 / h;
+        c[1] = -1.0 / h;
+        c[2] = 0.0;
+    }
+}
+
+std::vector<T> g_bdf_Xm;        // X_{n-1}
+double g_bdf_dtp = 0;           // the previous dt
+bool g_bdf_have = false;
+double g_bdf_last_norm = 0;     // |G| at the end of the last Newton solve
+int g_bdf_last_its = 0;
+
+void bdf_residual(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T t1, const T* X,
+                  const T* Xn, const T* Xm, const double* cf, const T* A, T* G, Work& w) {
+    const int nc = m->ncells;
+    const size_t n = (size_t)nc * NQ;
+    std::vector<T> R(n), S(n);
+    form_rhs(m, face_bc, p, o, t1, X, A, R.data(), w);
+    imex_source(nc, p, X, A, S.data());
+    for (size_t k = 0; k < n; ++k) {
+        const T xd = cf[0] * X[k] + cf[1] * Xn[k] + cf[2] * Xm[k];
+        G[k] = (xd - S[k]) - R[k];
+    }
+}
+
+void bdf_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T t0, T dt, T* X, const T* A,
+              Work& w) {
+    const size_t n = (size_t)m->ncells * NQ;
+    double cf[3];
+    bdf_coeffs(dt, g_bdf_dtp, g_bdf_have, cf);
+    const std::vector<T> Xn(X, X + n);
+    const std::vector<T>& Xm = g_bdf_have ? g_bdf_Xm : Xn;
+    const T t1 = t0 + dt;
+    std::vector<T> G(n), Gp(n), Gm(n), J(n * n), rhs(n), dx(n), Xt(n);
+    int it = 0;
+    for (; it < 30; ++it) {
+        bdf_residual(m, face_bc, p, o, t1, X, Xn.data(), Xm.data(), cf, A, G.data(), w);
+        for (size_t j = 0; j < n; ++j) {
+            const T hj = 1e-6 * (1.0 + std::fabs(X[j]));
+            Xt.assign(X, X + n);
+            Xt[j] = X[j] + hj;
+            bdf_residual(m, face_bc, p, o, t1, Xt.data(), Xn.data(), Xm.data(), cf, A, Gp.data(), w);
+            Xt[j] = X[j] - hj;
+            bdf_residual(m, face_bc, p, o, t1, Xt.data(), Xn.data(), Xm.data(), cf, A, Gm.data(), w);
+            for (size_t i = 0; i < n; ++i) J[i * n + j] = (Gp[i] - Gm[i]) / (2.0 * hj);
+        }
+        for (size_t i = 0; i < n; ++i) rhs[i] = -G[i];
+        if (!gepp(J.data(), rhs.data(), dx.data(), (int)n)) break;
+        T dmax = 0, xmax = 0;
+        for (size_t i = 0; i < n; ++i) {
+            X[i] += dx[i];
+            dmax = std::max(dmax, (T)std::fabs(dx[i]));
+            xmax = std::max(xmax, (T)std::fabs(X[i]));
+        }
+        if (dmax <= 1e-14 * (1.0 + xmax)) {
+            ++it;
+            break;
+        }
+    }
+    bdf_residual(m, face_bc, p, o, t1, X, Xn.data(), Xm.data(), cf, A, G.data(), w);
+    T gn = 0;
+    for (size_t i = 0; i < n; ++i) gn += G[i] * G[i];
+    g_bdf_last_norm = std::sqrt(gn);
+    g_bdf_last_its = it;
+    g_bdf_Xm = Xn;
+    g_bdf_dtp = dt;
+    g_bdf_have = true;
+}
+
+#endif  // ORACLE_TANGENT
+
 void ts_step(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, T t0, T dt, T* X,
              const T* A, Work& w) {
     const size_t n = (size_t)m->ncells * NQ;
@@ -925,10 +1011,22 @@ int orc_ts_step(const zdm_mesh* m, const int* face_bc, const double* p, int orde
 
 // solver.positivity / wet_dry_eps for the following calls (0 none, 1 mood, 2 zhang_shu); also
 // resets the MOOD override count
-// solver.time_integration of orc_run: 0 "splitting", 1 "imex"
+// solver.time_integration of orc_run: 0 "splitting", 1 "imex", 2 "implicit"
 int orc_set_time_integration(int ti) {
     g_time_integration = ti;
     return 0;
+}
+
+// the BDF coefficients of a step of size h after one of size h1 (have_prev = 0: BDF1)
+int orc_bdf_coeffs(double h, double h1, int have_prev, double* c) {
+    bdf_coeffs(h, h1, have_prev != 0, c);
+    return 0;
+}
+
+// |G| after the last implicit step's Newton solve, and its iterations
+double orc_bdf_last_residual(int* its) {
+    if (its) *its = g_bdf_last_its;
+    return g_bdf_last_norm;
 }
 
 // ARK2 on the scalar test equation y' = le y + li y (le explicit, li implicit): n steps of dt
@@ -979,6 +1077,7 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
     if (first) {
         enforce(m->ncells, p, X, A);
         dt = std::max(compute_time_step(m, p, X, A, cfl, nullptr), min_dt);
+        g_bdf_have = false;
     }
     const size_t n = (size_t)m->ncells * NQ;
     std::vector<T> Xs, Fm;
@@ -990,6 +1089,8 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
         if (g_positivity == 1) Xs.assign(X, X + n);
         if (g_time_integration == 1)
             imex_step(m, face_bc, p, o, 1, t0, dt0, X, A, w);
+        else if (g_time_integration == 2)
+            bdf_step(m, face_bc, p, o, t0, dt0, X, A, w);
         else
             ts_step(m, face_bc, p, o, t0, dt0, X, A, w);
         enforce(m->ncells, p, X, A);

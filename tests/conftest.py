@@ -157,7 +157,7 @@ class Oracle:
         t = np.array([time])
         d = np.array([dt])
         self.L.orc_set_positivity(self.POSITIVITY[positivity], float(wet_dry_eps))
-        self.L.orc_set_time_integration({"splitting": 0, "imex": 1}[time_integration])
+        self.L.orc_set_time_integration({"splitting": 0, "imex": 1, "implicit": 2}[time_integration])
         try:
             n = self.L.orc_run(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters,
                                gradient, int(well_balanced), cfl, t_end, min_dt, max_steps, dp(X),
@@ -227,6 +227,18 @@ class Oracle:
                                   dp(A), dp(P), dp(t), ctypes.byref(its), ctypes.byref(reason))
         return n, t[0], its.value, reason.value
 
+
+    def bdf_coeffs(self, h, h1, have_prev):
+        c = np.zeros(3)
+        self.L.orc_bdf_coeffs(ctypes.c_double(h), ctypes.c_double(h1), int(have_prev), dp(c))
+        return c
+
+    def bdf_last_residual(self):
+        """(|G| after the last implicit step's Newton solve, its Newton iterations)."""
+        self.L.orc_bdf_last_residual.restype = ctypes.c_double
+        its = ctypes.c_int()
+        r = self.L.orc_bdf_last_residual(ctypes.byref(its))
+        return r, its.value
 
     def ark2_linear(self, le, li, dt, n, y0=1.0):
         """The oracle's ARK2 step on y' = le y (explicit) + li y (implicit), n steps."""

@@ -81,11 +81,11 @@ def test_cli_refuses_what_it_does_not_provide(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     settings["solver"]["method"] = "muscl"
-    settings["solver"]["time_integration"] = "implicit"
+    settings["solver"]["time_integration"] = "crank_nicolson"
     sp.write_text(json.dumps(settings))
     r = subprocess.run([os.path.join(PKG_DIR, "zdm_cli_sme1"), "-settings", str(sp)],
                        capture_output=True, text=True, timeout=120)
-    assert r.returncode != 0 and "implicit" in r.stderr
+    assert r.returncode != 0 and "crank_nicolson" in r.stderr
 
 
 def read_vtu(path):

@@ -280,7 +280,17 @@ class _Options(ctypes.Structure):
                 ("wet_dry_eps", ctypes.c_double), ("time_integration", ctypes.c_int)]
 
 POSITIVITY = {"none": 0, "mood": 1, "zhang_shu": 2}
-TIME_INTEGRATION = {"splitting": 0, "imex": 1}
+TIME_INTEGRATION = {"splitting": 0, "imex": 1, "implicit": 2}
+
+
+class NewtonOptions(ctypes.Structure):
+    """zdm_newton_options: SNES / KSP settings of time_integration "implicit"."""
+    _fields_ = [("snes_atol", ctypes.c_double), ("snes_rtol", ctypes.c_double), ("snes_max_it", ctypes.c_int),
+                ("ksp_rtol", ctypes.c_double), ("ksp_restart", ctypes.c_int), ("ksp_max_it", ctypes.c_int)]
+
+    def __init__(self, snes_atol=1e-4, snes_rtol=1e-3, snes_max_it=20, ksp_rtol=1e-5, ksp_restart=30,
+                 ksp_max_it=10000):
+        super().__init__(snes_atol, snes_rtol, snes_max_it, ksp_rtol, ksp_restart, ksp_max_it)
 
 
 _model_libs = {}
@@ -298,7 +308,7 @@ SOLVER_SYMBOLS = [
     "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
     "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout", "zdm_troubled_cells",
     "zdm_set_launch", "zdm_get_launch", "zdm_autotune", "zdm_adjoint_setup_chorin",
-    "zdm_adjoint_krylov_its",
+    "zdm_adjoint_krylov_its", "zdm_implicit_setup", "zdm_implicit_stats",
 ]
 
 
@@ -351,6 +361,9 @@ def model_lib(model):
                                       _c_int_p, _c_dbl_p, ctypes.c_int]
     lib.zdm_adjoint_setup_chorin.argtypes = lib.zdm_adjoint_setup.argtypes
     lib.zdm_adjoint_krylov_its.argtypes = [V, ctypes.POINTER(ctypes.c_longlong)]
+    lib.zdm_implicit_setup.argtypes = [V, ctypes.POINTER(NewtonOptions)]
+    lib.zdm_implicit_stats.argtypes = [V, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.zdm_adjoint_forward.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     lib.zdm_adjoint_gradient.argtypes = [V, _c_dbl_p, _c_dbl_p]
     lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
@@ -577,6 +590,19 @@ class Solver:
         fn = self.lib.zdm_adjoint_setup_chorin if chorin else self.lib.zdm_adjoint_setup
         self._check(fn(self._h, int(nsteps), int(checkpoint_every), int(obs_every), len(g),
                        g.ctypes.data_as(_c_int_p), _dp(o) if o.size else None, int(bool(exact_limiter))))
+
+    def implicit_setup(self, **kw):
+        """Newton / Krylov settings of time_integration "implicit" (NewtonOptions fields)."""
+        o = NewtonOptions(**kw)
+        self._check(self.lib.zdm_implicit_setup(self._h, ctypes.byref(o)))
+
+    def implicit_stats(self):
+        """(Newton iterations, GMRES iterations, last step's Newton iterations, last reason)."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        c, d = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.zdm_implicit_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     def adjoint_krylov_its(self):
         """GMRES iterations of the transposed pressure solves since adjoint_setup."""

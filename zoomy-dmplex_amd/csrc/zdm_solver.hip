@@ -1069,6 +1069,7 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
 
 #include "krylov.inc"
 #include "imex.inc"
+#include "implicit.inc"
 #include "binomial.h"
 #include "adjoint.inc"
 #ifdef ZOOMY_MODEL_IS_VAM
@@ -1077,7 +1078,9 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
 #endif
 struct ChorinHost;
 struct AdjointHost;
+struct ImplicitHost;
 void adjoint_free(AdjointHost* ad);
+void implicit_free(ImplicitHost* im);
 
 }  // namespace
 
@@ -1113,6 +1116,8 @@ struct zdm_solver {
     hipGraph_t graph;                 // one captured time step (zdm_step_graph)
     hipGraphExec_t graph_exec;
     AdjointHost* adj;                 // calibration adjoint state (zdm_adjoint_*)
+    ImplicitHost* imp;                // time_integration "implicit": Newton-Krylov state
+    int step_err;                     // a host-driven step failed (the message is in zdm_last_error)
 };
 
 namespace {
@@ -1315,6 +1320,8 @@ int set_ctrl_rhs_time(zdm_solver* s, T t) {
     return 0;
 }
 
+#include "implicit_host.inc"
+
 // all control blocks (one per member) to the host
 int read_ctrls(zdm_solver* s, std::vector<Ctrl>& cs) {
     cs.resize(s->nctrl);
@@ -1389,10 +1396,12 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     if (opts->positivity != 0 && opts->well_balanced)
         return fail("positivity and the well_balanced extension cannot be combined");
     if (opts->positivity != 0 && NQ < 2) return fail("positivity needs the [b, h, ...] state layout");
-    if (opts->time_integration < 0 || opts->time_integration > 1)
-        return fail("time_integration must be 0 (splitting) or 1 (imex)");
-    if (opts->time_integration == 1 && nmem > 0) return fail("time_integration imex: not available for an ensemble");
-    if (opts->time_integration == 1 && NQ < 2) return fail("time_integration imex needs the [b, h, ...] state layout");
+    if (opts->time_integration < 0 || opts->time_integration > 2)
+        return fail("time_integration must be 0 (splitting), 1 (imex) or 2 (implicit)");
+    if (opts->time_integration != 0 && nmem > 0)
+        return fail("time_integration imex / implicit: not available for an ensemble");
+    if (opts->time_integration != 0 && NQ < 2)
+        return fail("time_integration imex / implicit needs the [b, h, ...] state layout");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
 
@@ -1655,6 +1664,7 @@ int zdm_solver_destroy(zdm_solver* s) {
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
     chorin_free(s->ch);
     adjoint_free(s->adj);
+    implicit_free(s->imp);
     hipFree(s->arena);
     hipStreamDestroy(s->stream);
     delete s;
@@ -1728,6 +1738,7 @@ int zdm_initialize(zdm_solver* s) {
     std::memset(c0.data(), 0, sizeof(Ctrl) * c0.size());
     for (Ctrl& c : c0) c.active = 1;
     HIPCHK(hipMemcpyAsync(s->ctrl, c0.data(), sizeof(Ctrl) * c0.size(), hipMemcpyHostToDevice, s->stream));
+    if (s->imp) s->imp->have_prev = false;   // BDF history restarts with BDF1
     post(s, 1);
     finalize(s, 1);
     HIPCHK(hipGetLastError());
@@ -1809,6 +1820,11 @@ static void enqueue_step(zdm_solver* s) {
         enqueue_imex_step(s);
         return;
     }
+    if (s->opt.time_integration == 2) {
+        // host-driven Newton loop (synchronises): not capturable
+        if (implicit_step(s)) s->step_err = 1;
+        return;
+    }
     const int order = s->opt.order, st = s->opt.ssp_stages;
     const bool mood = s->opt.positivity == 1;
     if (mood) (void)hipMemcpyAsync(s->Xs, s->X, sizeof(T) * (size_t)NQ * s->ld, hipMemcpyDeviceToDevice, s->stream);
@@ -1845,6 +1861,10 @@ int zdm_step(zdm_solver* s, int n) {
         if (s->timing && s->evpool.size() > 4096 && s->evused + 64 > s->evpool.size())
             if (resolve_events(s)) return -1;
         enqueue_step(s);
+        if (s->step_err) {
+            s->step_err = 0;
+            return -1;
+        }
         HIPCHK(hipGetLastError());
     }
     return 0;
@@ -1852,6 +1872,7 @@ int zdm_step(zdm_solver* s, int n) {
 
 int zdm_step_graph(zdm_solver* s, int n) {
     if (!s) return fail("null solver");
+    if (s->opt.time_integration == 2) return fail("zdm_step_graph: the implicit step synchronises (Newton) and is not captured");
     if (!s->graph_exec) {
         const bool was = s->timing;
         s->timing = false;
@@ -2235,6 +2256,38 @@ int zdm_adjoint_setup_chorin(zdm_solver* s, int nsteps, int checkpoint_every, in
     return 0;
 }
 
+int zdm_implicit_setup(zdm_solver* s, const zdm_newton_options* opts) {
+    if (!s) return fail("null solver");
+    if (s->opt.time_integration != 2) return fail("zdm_implicit_setup: the solver's time_integration is not implicit");
+    const zdm_newton_options o = opts ? *opts : newton_defaults();
+    const bool had = s->imp && s->imp->have_prev;
+    const T dtp = s->imp ? s->imp->dt_prev : 0;
+    std::vector<T> keep;
+    if (had) {
+        keep.resize((size_t)NQ * s->ld);
+        HIPCHK(hipMemcpy(keep.data(), s->imp->Xm, sizeof(T) * keep.size(), hipMemcpyDeviceToHost));
+    }
+    if (implicit_build(s, o)) return -1;
+    if (had) {
+        // keep the BDF history across a change of settings
+        HIPCHK(hipMemcpy(s->imp->Xm, keep.data(), sizeof(T) * keep.size(), hipMemcpyHostToDevice));
+        s->imp->have_prev = true;
+        s->imp->dt_prev = dtp;
+    }
+    return 0;
+}
+
+int zdm_implicit_stats(zdm_solver* s, long long* newton_its, long long* ksp_its, int* last_newton,
+                       int* last_reason) {
+    if (!s) return fail("null solver");
+    const ImplicitHost* im = s->imp;
+    if (newton_its) *newton_its = im ? im->newton_its : 0;
+    if (ksp_its) *ksp_its = im ? im->ksp_its : 0;
+    if (last_newton) *last_newton = im ? im->last_newton : 0;
+    if (last_reason) *last_reason = im ? im->last_reason : 0;
+    return 0;
+}
+
 int zdm_adjoint_krylov_its(zdm_solver* s, long long* its) {
     if (!s || !s->adj || !its) return fail("zdm_adjoint_krylov_its: bad arguments");
     *its = s->adj->its_t;
@@ -2275,6 +2328,7 @@ int zdm_get_launch(zdm_solver* s, int* block_size, int* cells_per_thread) {
 int zdm_autotune(zdm_solver* s, int steps, int* block_size, int* cells_per_thread, double* ms) {
     if (!s) return fail("null solver");
     if (steps < 1) return fail("zdm_autotune: steps must be >= 1");
+    if (s->opt.time_integration == 2) return fail("zdm_autotune: not for the implicit step");
     static const int cand[9][2] = {{256, 1}, {128, 1}, {64, 1}, {256, 2}, {128, 2}, {64, 2}, {256, 4}, {128, 4}, {64, 4}};
     const size_t bX = sizeof(T) * (size_t)NQ * s->ld, bA = sizeof(T) * (size_t)NA * s->ld,
                  bC = sizeof(Ctrl) * (size_t)s->nctrl;
