@@ -78,11 +78,13 @@ struct SmeModel2D {
     static const std::vector<std::string> parameter_names() { return {"g", "nu", "lambda"}; }
     static const std::vector<T> default_parameters() { return {9.81, 1.0e-3, 0.1}; }
 
-    // Kurganov-Petrova regularised inverse depth: sqrt(2) max(0,h) / sqrt(h^4 + max(eps,h)^4)
+    // Kurganov-Petrova regularised inverse depth: sqrt(2) max(0,h) / sqrt(h^4 + max(eps,h)^4),
+    // which is 1/h wherever h >= eps (the wet case: one division instead of a square root and a
+    // division)
     PORTABLE_FN static inline T hinv_kp(T h, T eps) {
-        const T hm = std::max(eps, h);
-        const T h2 = h * h, hm2 = hm * hm;
-        return (T)1.4142135623730951 * std::max((T)0, h) / std::sqrt(h2 * h2 + hm2 * hm2);
+        if (h >= eps) return (T)1 / h;
+        const T h2 = h * h, e2 = eps * eps;
+        return (T)1.4142135623730951 * std::max((T)0, h) / std::sqrt(h2 * h2 + e2 * e2);
     }
 
     PORTABLE_FN static inline SimpleArray<T, n_dof_q> source(const T* Q, const T* Qaux, const T* p,
@@ -444,15 +446,14 @@ struct SmeNumerics2D {
 
     // d hinv_kp(h, eps) / dh times bres
     PORTABLE_FN static inline T hinv_kp_vjp(T h, T eps, T bres) {
-        const T hm = std::max(eps, h);
-        const T h2 = h * h, hm2 = hm * hm;
-        const T den = std::sqrt(h2 * h2 + hm2 * hm2);
+        if (h >= eps) return -(bres / (h * h));   // d(1/h)/dh
+        const T h2 = h * h, e2 = eps * eps;
+        const T den = std::sqrt(h2 * h2 + e2 * e2);
         const T c = (T)1.4142135623730951;
         const T bnum = c * bres / den;
         const T bden = -(c * std::max((T)0, h)) * bres / (den * den);
         const T barg = bden / ((T)2 * den);
         T bh = (T)2 * h * ((T)2 * h2 * barg);
-        if (eps < h) bh += (T)2 * hm * ((T)2 * hm2 * barg);
         if ((T)0 < h) bh += bnum;
         return bh;
     }
