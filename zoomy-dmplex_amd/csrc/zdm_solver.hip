@@ -256,10 +256,12 @@ __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
 // of the cell (Reconstruction.hpp:203-243; GRAD 3: WBPositivityReconstructor, 256-287).
 // put(k, i, v) receives component i of face slot k's state; the clamped aux is returned in ac.
 // wet_eps / zs_theta: GRAD 3 only (wet_dry_eps; 1 = apply the Zhang-Shu theta).
-template <int MAXF, int GRAD, typename Put>
+// AMODE (limited linear reconstruction, GRAD 1 / 2): 0 computes the face limiter values alpha,
+// 1 computes and stores them in abuf[k ld + c], 2 takes them from abuf (a frozen limiter)
+template <int MAXF, int GRAD, typename Put, int AMODE = 0>
 __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, const T* __restrict__ X,
                                            const T* __restrict__ Ag, int c, int use_limiters, T* ac, Put put,
-                                           T wet_eps = 0, int zs_theta = 0) {
+                                           T wet_eps = 0, int zs_theta = 0, T* abuf = nullptr) {
     constexpr bool GG = GRAD == 1 || GRAD == 3;
     const int ld = m.ld;
     T q[NQ], qc[NQ];
@@ -472,7 +474,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
                 for (int d = 0; d < DIM; ++d) delta[i] += g[i * DIM + d] * dx[d];
                 fin[i] = 1.0;
             }
-            if (use_limiters) {
+            if (AMODE != 2 && use_limiters) {
                 // vacuum limiter: first order next to (nearly) dry cells
                 if (NQ > 1 && lo[1] < 1e-2) {
 #pragma unroll
@@ -503,6 +505,8 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
 #pragma unroll
             for (int i = 0; i < NQ; ++i)
                 if (alpha > fin[i]) alpha = fin[i];
+            if constexpr (AMODE == 1) abuf[(size_t)k * ld + c] = alpha;
+            if constexpr (AMODE == 2) alpha = abuf[(size_t)k * ld + c];
 #pragma unroll
             for (int i = 0; i < NQ; ++i) put(k, i, qc[i] + alpha * delta[i]);
         }
@@ -535,6 +539,21 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
         if (L.c >= m.ncells) return;
         body(L.c);
     }
+}
+
+// k_prep of one mesh (single-cell launch shape) with the limiter values stored (AMODE 1) or
+// taken from abuf (AMODE 2), for the matrix-free Jacobian of the implicit step
+template <int MAXF, int GRAD, int AMODE>
+__global__ __launch_bounds__(BLOCK) void k_prep_alpha(DevMesh m, Params P, const T* __restrict__ X,
+                                                      const T* __restrict__ Ag, T* __restrict__ As, T* __restrict__ QF,
+                                                      int use_limiters, T* __restrict__ abuf) {
+    const int c = cell_index();
+    if (c >= m.ncells) return;
+    const int ld = m.ld;
+    T ac[NA];
+    auto put = [&](int k, int i, T v) { QF[((size_t)k * NQ + i) * ld + c] = v; };
+    recon_cell<MAXF, GRAD, decltype(put), AMODE>(m, P, X, Ag, c, use_limiters, ac, put, (T)0, 0, abuf);
+    store_soa<NA>(As, ld, c, ac);
 }
 
 // ------------------------------------------------------------------------------ k_flux
