@@ -279,6 +279,60 @@ def run_c3adj(zdm, cases, args):
     print(json.dumps(out))
 
 
+def run_c2_time_integration(zdm, cases, args):
+    """The C2 workload under the IMEX (ARK2) or fully implicit (BDF2, Newton-Krylov) strategy:
+    throughput, the kernels' share, and for the implicit step the Newton / GMRES counts."""
+    n = args.grid
+    mesh = zdm.Mesh.tri_grid(n, n, renumber=args.order)
+    info = zdm.ModelInfo("sme2")
+    X = cases.c2_state(mesh, info.n_q, seed=2)
+    ti = args.time_integration
+    s = zdm.Solver("sme2", mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e9, source=True,
+                   gradient="green_gauss", use_limiters=args.limiter, time_integration=ti)
+    s.upload(X, None)
+    s.initialize()
+    s.step(args.warmup)
+    s.synchronize()
+    n0 = s.implicit_stats() if ti == "implicit" else None
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    Xg, _ = s.download()
+    if not np.all(np.isfinite(Xg)):
+        raise RuntimeError("non-finite state after the benchmark")
+    ncells = mesh.ncells
+    cfg = {
+        "workload": "SME level 2 + implicit slip source, %dx%d right-triangle grid (%d cells), MUSCL "
+                    "(limiter %s), time_integration %s, CFL 0.45" % (n, n, ncells, "on" if args.limiter else "off", ti),
+        "cells": ncells,
+        "time_integration": ti,
+    }
+    if ti == "implicit":
+        nw, ksp, last, reason = s.implicit_stats()
+        cfg["newton_its_per_step"] = round((nw - n0[0]) / args.steps, 2)
+        cfg["gmres_its_per_step"] = round((ksp - n0[1]) / args.steps, 2)
+        cfg["last_snes_reason"] = reason
+    out = {
+        "metric": "Mcell-steps/s (SME L2, FP64)",
+        "value": round(ncells * args.steps / elapsed / 1e6, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": cfg,
+        "roofline": None,
+    }
+    s.close()
+    print(json.dumps(out))
+
+
 def c4_members(zdm, cases, count):
     meshes, states = [], []
     for i in range(count):
@@ -484,6 +538,10 @@ def main():
     ap.add_argument("--order", default="rcm", choices=["hilbert", "rcm", "none"],
                     help="cell renumbering of the mesh service")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--time-integration", default="splitting", choices=["splitting", "imex", "implicit"],
+                    help="c2 strategy: splitting (the headline), imex (ARK2), implicit (BDF2 Newton-Krylov)")
+    ap.add_argument("--no-limiter", dest="limiter", action="store_false",
+                    help="c2 with imex / implicit: unlimited reconstruction")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d refused)"
@@ -502,6 +560,8 @@ def main():
         return run_c3adj(zdm, cases, args)
     if args.workload == "c4":
         return run_c4(zdm, cases, args)
+    if args.time_integration != "splitting":
+        return run_c2_time_integration(zdm, cases, args)
 
     n = args.grid
     mesh = zdm.Mesh.tri_grid(n, n, renumber=args.order)
