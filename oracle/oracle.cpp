@@ -681,6 +681,10 @@ void ssp_or_euler(int order, int s, size_t n, T t0, T dt, T* X, RHS&& rhs) {
 // available here: which member of the family its "2e" registers is unpinned.
 //   Y1 = X;  Yi = X + dt sum_{j<i} (a_ij G_j + at_ij S_j) + dt g S(Yi);  X+ = X + dt sum b_i (G_i + S_i)
 // rhs(t, U, F) = G; src(U, S) = S; solve(dg, Z, Y): Y - dg S(Y) = Z.  S_i = (Y_i - Z_i)/(dt g).
+// Evaluated with running sums, each stage's terms added once they are known:
+//   Z2 = X + dt (a21 G1 + at21 S1),  P3 = X + dt (a31 G1 + at31 S1 + at32 S2),
+//   F = X + dt (b1 (G1 + S1) + b2 S2),  Z3 = P3 + dt a32 G2,  F += dt (b2 G2 + b3 S3),
+//   X+ = F + dt b3 G3
 namespace ark2o {
 const double R2 = 1.41421356237309504880;
 const double G = 1.0 - 1.0 / R2, D = 1.0 / (2.0 * R2);
@@ -692,19 +696,26 @@ const double B1 = D, B2 = D, B3 = G, C2 = 2.0 * G;
 template <typename RHS, typename SRC, typename SOLVE>
 void ark2_step(size_t n, T t0, T dt, T* X, RHS&& rhs, SRC&& src, SOLVE&& solve) {
     using namespace ark2o;
-    std::vector<T> G1(n), G2(n), G3(n), S1(n), S2(n), S3(n), Y(n), Z(n);
+    std::vector<T> Gv(n), S1(n), Y(n), Z(n), P3(n), F(n);
     const T dg = dt * G;
-    rhs(t0, X, G1.data());
+    rhs(t0, X, Gv.data());
     src(X, S1.data());
-    for (size_t k = 0; k < n; ++k) Z[k] = X[k] + dt * (A21 * G1[k] + AT21 * S1[k]);
+    for (size_t k = 0; k < n; ++k) Z[k] = X[k] + dt * (A21 * Gv[k] + AT21 * S1[k]);
     solve(dg, Z.data(), Y.data());
-    for (size_t k = 0; k < n; ++k) S2[k] = (Y[k] - Z[k]) / dg;
-    rhs(t0 + C2 * dt, Y.data(), G2.data());
-    for (size_t k = 0; k < n; ++k) Z[k] = X[k] + dt * (A31 * G1[k] + A32 * G2[k] + AT31 * S1[k] + AT32 * S2[k]);
+    for (size_t k = 0; k < n; ++k) {
+        const T s2 = (Y[k] - Z[k]) / dg;
+        P3[k] = X[k] + dt * (A31 * Gv[k] + AT31 * S1[k] + AT32 * s2);
+        F[k] = X[k] + dt * (B1 * (Gv[k] + S1[k]) + B2 * s2);
+    }
+    rhs(t0 + C2 * dt, Y.data(), Gv.data());
+    for (size_t k = 0; k < n; ++k) Z[k] = P3[k] + dt * (A32 * Gv[k]);
     solve(dg, Z.data(), Y.data());
-    for (size_t k = 0; k < n; ++k) S3[k] = (Y[k] - Z[k]) / dg;
-    rhs(t0 + dt, Y.data(), G3.data());
-    for (size_t k = 0; k < n; ++k) X[k] = X[k] + dt * (B1 * (G1[k] + S1[k]) + B2 * (G2[k] + S2[k]) + B3 * (G3[k] + S3[k]));
+    for (size_t k = 0; k < n; ++k) {
+        const T s3 = (Y[k] - Z[k]) / dg;
+        F[k] = F[k] + dt * (B2 * Gv[k] + B3 * s3);
+    }
+    rhs(t0 + dt, Y.data(), Gv.data());
+    for (size_t k = 0; k < n; ++k) X[k] = F[k] + dt * (B3 * Gv[k]);
 }
 
 // the IFunction's source (AddImplicitSourceToResidual, ModularSolver.hpp:82-124): per cell on

@@ -31,7 +31,7 @@ def run_cli(model, settings, tmp_path, *extra):
 
 
 @pytest.mark.parametrize("positivity,ti", [("none", "splitting"), ("mood", "splitting"), ("zhang_shu", "splitting"),
-                                           ("none", "imex"), ("mood", "imex")])
+                                           ("none", "imex"), ("mood", "imex"), ("none", "implicit")])
 def test_cli_muscl_equals_python_solver(tmp_path, positivity, ti):
     mesh = zdm.Mesh.read_msh(MESH)
     xy = mesh.cell_centroid[:, :2]

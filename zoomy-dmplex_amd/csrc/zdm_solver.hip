@@ -559,11 +559,74 @@ __global__ __launch_bounds__(BLOCK) void k_prep_alpha(DevMesh m, Params P, const
 // ------------------------------------------------------------------------------ k_flux
 enum { MODE_RHS = 0, MODE_EULER = 1, MODE_STAGE = 2, MODE_FINAL = 3 };
 
-// The face loop of ComputeFluxes for cell c (TransportStep.hpp:383-565): per face slot in
-// ascending order the face aux, flux and fluctuations of (own, neighbour) face states and the
-// -D A -/+ F A accumulation, boundary ghosts, the optional well-balanced cell term; then / vol.
+// One face slot of the face loop of ComputeFluxes (TransportStep.hpp:383-565) for a slot with
+// f >= 0 and area > 1e-15: the face aux, flux and fluctuations of the (own qo, neighbour qn with
+// clamped aux an) face states and the -D A -/+ F A accumulation into fc, or the boundary ghost
+// (nb < 0), then the optional well-balanced cell term.  meta = the slot's cmeta entry (read only
+// when nb >= 0); ac = the cell's clamped aux, qcell = its clamped state (well-balanced term only).
+__device__ __forceinline__ void flux_face(const DevMesh& m, const Params& P, int f, int nb, int meta, T area,
+                                          const T* nh, const T* xf, const T* qo, const T* qn, const T* an,
+                                          const T* ac, T time, int wb, const T* qcell, T* fc) {
+    if (nb >= 0) {
+        const bool right = (meta >> 2) & 1;
+        const T* qL = right ? qn : qo;
+        const T* qR = right ? qo : qn;
+        const T* aLc = right ? an : ac;
+        const T* aRc = right ? ac : an;
+        auto aL = M::update_aux_variables(qL, aLc, P.p, 0.0, xf);
+        auto aR = M::update_aux_variables(qR, aRc, P.p, 0.0, xf);
+        T F[NQ], D[2 * NQ];
+        face_flux<NUM, T>(qL, qR, aL.data, aR.data, P.p, nh, F, D);
+        if (right) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                fc[i] -= D[i] * area;
+                fc[i] += F[i] * area;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                fc[i] -= D[NQ + i] * area;
+                fc[i] -= F[i] * area;
+            }
+        }
+    } else {
+        const int bc = m.fbc[f];
+        if (bc >= 0) {
+            auto aL = M::update_aux_variables(qo, ac, P.p, time, xf);
+            auto qR = M::boundary_conditions(bc, qo, aL.data, P.p, nh, xf, time, 0.0);
+            auto aR = M::update_aux_variables(qR.data, aL.data, P.p, time, xf);
+            T F[NQ], D[2 * NQ];
+            face_flux<NUM, T>(qo, qR.data, aL.data, aR.data, P.p, nh, F, D);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                fc[i] -= D[NQ + i] * area;
+                fc[i] -= F[i] * area;
+            }
+        }
+    }
+    if constexpr (HasCellSource<T>::value) {
+        if (wb) {
+            const bool right = nb >= 0 && ((meta >> 2) & 1);
+            const T no[3] = {right ? -nh[0] : nh[0], right ? -nh[1] : nh[1], 0};
+            auto S = M::hydrostatic_cell_source(qo, qcell, P.p, no);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) fc[i] += S[i] * area;
+        }
+    }
+}
+
+__device__ __forceinline__ void flux_divide_volume(const DevMesh& m, int c, T* fc) {
+    const T vol = m.vol[c];
+    if (vol > 1e-15) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) fc[i] /= vol;
+    }
+}
+
+// The face loop of ComputeFluxes for cell c, slot by slot in ascending order, then / vol.
 // own(k, qo) loads face slot k's own state, nbr(k, nb, ks, qn, an) the neighbour's state and
-// clamped aux; ac = the cell's clamped aux, qcell = its clamped state (well-balanced term only).
+// clamped aux.
 template <int MAXF, typename Own, typename Nbr>
 __device__ __forceinline__ void flux_cell(const DevMesh& m, const Params& P, int c, T time, const T* ac, int wb,
                                           const T* qcell, Own own, Nbr nbr, T* fc) {
@@ -579,65 +642,69 @@ __device__ __forceinline__ void flux_cell(const DevMesh& m, const Params& P, int
         const int nb = m.cnbr[(size_t)k * ld + c];
         const T nh[3] = {m.fnhx[f], DIM > 1 ? m.fnhy[f] : (T)0, 0};
         const T xf[3] = {m.fcx[f], DIM > 1 ? m.fcy[f] : (T)0, 0};
-        T qo[NQ];
+        T qo[NQ], qn[NQ], an[NA];
         own(k, qo);
+        int meta = 0;
         if (nb >= 0) {
-            const int meta = m.cmeta[(size_t)k * ld + c];
-            const int ks = meta & 3;
-            const bool right = (meta >> 2) & 1;
-            T qn[NQ], an[NA];
-            nbr(k, nb, ks, qn, an);
-            const T* qL = right ? qn : qo;
-            const T* qR = right ? qo : qn;
-            const T* aLc = right ? an : ac;
-            const T* aRc = right ? ac : an;
-            auto aL = M::update_aux_variables(qL, aLc, P.p, 0.0, xf);
-            auto aR = M::update_aux_variables(qR, aRc, P.p, 0.0, xf);
-            T F[NQ], D[2 * NQ];
-            face_flux<NUM, T>(qL, qR, aL.data, aR.data, P.p, nh, F, D);
-            if (right) {
-#pragma unroll
-                for (int i = 0; i < NQ; ++i) {
-                    fc[i] -= D[i] * area;
-                    fc[i] += F[i] * area;
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < NQ; ++i) {
-                    fc[i] -= D[NQ + i] * area;
-                    fc[i] -= F[i] * area;
-                }
-            }
-        } else {
-            const int bc = m.fbc[f];
-            if (bc >= 0) {
-                auto aL = M::update_aux_variables(qo, ac, P.p, time, xf);
-                auto qR = M::boundary_conditions(bc, qo, aL.data, P.p, nh, xf, time, 0.0);
-                auto aR = M::update_aux_variables(qR.data, aL.data, P.p, time, xf);
-                T F[NQ], D[2 * NQ];
-                face_flux<NUM, T>(qo, qR.data, aL.data, aR.data, P.p, nh, F, D);
-#pragma unroll
-                for (int i = 0; i < NQ; ++i) {
-                    fc[i] -= D[NQ + i] * area;
-                    fc[i] -= F[i] * area;
-                }
-            }
+            meta = m.cmeta[(size_t)k * ld + c];
+            nbr(k, nb, meta & 3, qn, an);
         }
-        if constexpr (HasCellSource<T>::value) {
-            if (wb) {
-                const bool right = nb >= 0 && ((m.cmeta[(size_t)k * ld + c] >> 2) & 1);
-                const T no[3] = {right ? -nh[0] : nh[0], right ? -nh[1] : nh[1], 0};
-                auto S = M::hydrostatic_cell_source(qo, qcell, P.p, no);
-#pragma unroll
-                for (int i = 0; i < NQ; ++i) fc[i] += S[i] * area;
-            }
-        }
+        flux_face(m, P, f, nb, meta, area, nh, xf, qo, qn, an, ac, time, wb, qcell, fc);
     }
-    const T vol = m.vol[c];
-    if (vol > 1e-15) {
+    flux_divide_volume(m, c, fc);
+}
+
+// Inputs of one face slot of cell c for the k_flux gather.  The topology of all slots is read
+// first; every load of a slot then uses an in-bounds index whatever the slot holds (no face:
+// face 0; boundary face: the cell itself, slot 0), so a slot's loads issue together: one memory
+// round trip per slot in place of the dependent chain cface -> farea -> cnbr -> cmeta ->
+// neighbour state.  (Prefetching the next slot during this one's flux needs 156 VGPRs, one wave
+// per SIMD less, and measures slower.)
+struct FaceIn {
+    T area, nh[3], xf[3];
+    T qo[NQ], qn[NQ], an[NA];
+};
+
+template <int MAXF>
+__device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params& P, int c, T time, const T* ac,
+                                                 int wb, const T* qcell, const T* __restrict__ QF,
+                                                 const T* __restrict__ As, T* fc) {
+    const int ld = m.ld;
+    int fk[MAXF], nk[MAXF], mk[MAXF];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) fc[i] /= vol;
+    for (int k = 0; k < MAXF; ++k) {
+        fk[k] = m.cface[(size_t)k * ld + c];
+        nk[k] = m.cnbr[(size_t)k * ld + c];
+        mk[k] = m.cmeta[(size_t)k * ld + c];
     }
+    auto load = [&](int k, FaceIn& in) {
+        const int f = fk[k] >= 0 ? fk[k] : 0;
+        const int nb = nk[k] >= 0 ? nk[k] : c;
+        const int ks = nk[k] >= 0 ? (mk[k] & 3) : 0;
+        in.area = m.farea[f];
+        in.nh[0] = m.fnhx[f];
+        in.nh[1] = DIM > 1 ? m.fnhy[f] : (T)0;
+        in.nh[2] = 0;
+        in.xf[0] = m.fcx[f];
+        in.xf[1] = DIM > 1 ? m.fcy[f] : (T)0;
+        in.xf[2] = 0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) in.qo[i] = QF[((size_t)k * NQ + i) * ld + c];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) in.qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
+        load_soa<NA>(As, ld, nb, in.an);
+    };
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) fc[i] = 0;
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) {
+        FaceIn in;
+        load(k, in);
+        if (fk[k] >= 0 && in.area > 1e-15)
+            flux_face(m, P, fk[k], nk[k], mk[k], in.area, in.nh, in.xf, in.qo, in.qn, in.an, ac, time, wb, qcell,
+                      fc);
+    }
+    flux_divide_volume(m, c, fc);
 }
 
 // Out = the stage update of the cell from its right-hand side fc (MODE_RHS: fc itself)
@@ -684,18 +751,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
                 for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
             }
         }
-        flux_cell<MAXF>(
-            m, P, c, time, ac, wb, qcell,
-            [&](int k, T* qo) {
-    #pragma unroll
-                for (int i = 0; i < NQ; ++i) qo[i] = QF[((size_t)k * NQ + i) * ld + c];
-            },
-            [&](int k, int nb, int ks, T* qn, T* an) {
-    #pragma unroll
-                for (int i = 0; i < NQ; ++i) qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
-                load_soa<NA>(As, ld, nb, an);
-            },
-            fc);
+        flux_cell_gather<MAXF>(m, P, c, time, ac, wb, qcell, QF, As, fc);
         stage_update<MODE>(ctrl, c, ld, fc, In, Out, X0);
     };
     if constexpr (MC) {
@@ -1120,7 +1176,7 @@ struct zdm_solver {
     DevMesh dm;
     T *X, *W, *F, *A, *As, *Atmp, *QF, *partials, *stage_aos;
     T* Xs;                            // MOOD: the state at the start of the step
-    T* Im;                            // IMEX: stage right-hand sides G1..3, sources S1..3, stage state
+    T* Im;                            // IMEX: stage right-hand side, stage state, running sums P3, F
     double cfl_in;                    // solver.cfl as given (opt.cfl: capped for zhang_shu)
     Ctrl *ctrl, *ctrl_rhs;
     bool timing;
@@ -1542,7 +1598,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     size_t oX = take(sQ), oW = take(sQ), oF = take(sQ), oA = take(sA), oAs = take(sA), oAt = take(sA);
     size_t oQF = take(sizeof(T) * (size_t)MFa * NQ * ld);
     size_t oXs = take(s->opt.positivity == 1 ? sQ : 8);   // MOOD: the saved start-of-step state
-    size_t oIm = take(s->opt.time_integration == 1 ? 7 * sQ : 8);   // IMEX: G1..3, S1..3, stage state
+    size_t oIm = take(s->opt.time_integration == 1 ? 4 * sQ : 8);   // IMEX: G, stage state, P3, F
     size_t oPart = take(sizeof(T) * (size_t)s->maxblocks);
     size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
     size_t oCtrl = take(sizeof(Ctrl) * s->nctrl), oCtrlR = take(sizeof(Ctrl) * s->nctrl);
@@ -1795,13 +1851,14 @@ static int resolve_events(zdm_solver* s) {
 }
 
 // the launches of one IMEX step (imex.inc): three transport right-hand sides, the per-cell
-// implicit stages, the combination; then constraints [with the MOOD override] and the CFL dt as
-// after any step (no split source)
+// implicit stages with the running sums, the last term; then constraints [with the MOOD
+// override] and the CFL dt as after any step (no split source)
 static void enqueue_imex_step(zdm_solver* s) {
     const size_t nQ = (size_t)NQ * s->ld;
     T* G = s->Im;
-    T* S = s->Im + 3 * nQ;
-    T* Y = s->Im + 6 * nQ;
+    T* Y = s->Im + nQ;
+    T* P3 = s->Im + 2 * nQ;
+    T* F = s->Im + 3 * nQ;
     const int src = s->opt.source;
     dim3 g(s->ncb), b(BLOCK);
     if (s->opt.positivity == 1)
@@ -1812,19 +1869,17 @@ static void enqueue_imex_step(zdm_solver* s) {
     record(s, 0);
     flux(s, s->ctrl, MODE_RHS, 0, s->X, G, s->A);
     record(s, 1);
-    k_imex_s1<<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, S, src);
-    k_imex_stage<2><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, G, nullptr, S, nullptr, Y, S + nQ, src);
+    k_imex_stage2<<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, G, Y, P3, F, src);
     prep(s, s->ctrl, Y, s->A);
     record(s, 0);
-    flux(s, s->ctrl, MODE_RHS, 1, Y, G + nQ, s->A);
+    flux(s, s->ctrl, MODE_RHS, 1, Y, G, s->A);
     record(s, 1);
-    k_imex_stage<3><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, G, G + nQ, S, S + nQ, Y, S + 2 * nQ,
-                                             src);
+    k_imex_stage3<<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->A, G, P3, F, Y, src);
     prep(s, s->ctrl, Y, s->A);
     record(s, 0);
-    flux(s, s->ctrl, MODE_RHS, 2, Y, G + 2 * nQ, s->A);
+    flux(s, s->ctrl, MODE_RHS, 2, Y, G, s->A);
     record(s, 1);
-    k_imex_final<<<g, b, 0, s->stream>>>(s->dm, s->ctrl, s->X, G, S, nQ);
+    k_imex_final<<<g, b, 0, s->stream>>>(s->dm, s->ctrl, s->X, F, G);
     if (s->opt.positivity == 1)
         post_mood(s);
     else
