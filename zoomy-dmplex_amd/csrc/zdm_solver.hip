@@ -104,6 +104,14 @@ using M = Model<T>;
 using NUM = Numerics<T>;
 constexpr int NQ = M::n_dof_q;
 constexpr int NA = M::n_dof_qaux;
+// Face states QF: per face slot k, NQP / 2 planes of 16-byte pairs (components 2j, 2j+1 of cell
+// c side by side), NQP = NQ rounded up to even: k_prep's stores and k_flux's own-state loads
+// are 16-byte vector accesses contiguous across the wavefront, and a neighbour's face state is
+// NQP / 2 vector loads.
+constexpr int NQP = (NQ + 1) & ~1;
+__host__ __device__ __forceinline__ size_t qf_idx(int ld, int k, int c, int i) {
+    return (((size_t)k * (NQP / 2) + (i >> 1)) * ld + c) * 2 + (i & 1);
+}
 constexpr int DIM = M::dimension;
 constexpr int NPAR = M::n_parameters;
 constexpr int MAXPAR = 16;
@@ -525,7 +533,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
     auto body = [&](int c) {
         T ac[NA];
         recon_cell<MAXF, GRAD>(
-            m, P, X, Ag, c, use_limiters, ac, [&](int k, int i, T v) { QF[((size_t)k * NQ + i) * ld + c] = v; },
+            m, P, X, Ag, c, use_limiters, ac, [&](int k, int i, T v) { QF[qf_idx(ld, k, c, i)] = v; },
             wet_eps, zs_theta);
         store_soa<NA>(As, ld, c, ac);
     };
@@ -551,7 +559,7 @@ __global__ __launch_bounds__(BLOCK) void k_prep_alpha(DevMesh m, Params P, const
     if (c >= m.ncells) return;
     const int ld = m.ld;
     T ac[NA];
-    auto put = [&](int k, int i, T v) { QF[((size_t)k * NQ + i) * ld + c] = v; };
+    auto put = [&](int k, int i, T v) { QF[qf_idx(ld, k, c, i)] = v; };
     recon_cell<MAXF, GRAD, decltype(put), AMODE>(m, P, X, Ag, c, use_limiters, ac, put, (T)0, 0, abuf);
     store_soa<NA>(As, ld, c, ac);
 }
@@ -689,9 +697,9 @@ __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params&
         in.xf[1] = DIM > 1 ? m.fcy[f] : (T)0;
         in.xf[2] = 0;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) in.qo[i] = QF[((size_t)k * NQ + i) * ld + c];
+        for (int i = 0; i < NQ; ++i) in.qo[i] = QF[qf_idx(ld, k, c, i)];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) in.qn[i] = QF[((size_t)ks * NQ + i) * ld + nb];
+        for (int i = 0; i < NQ; ++i) in.qn[i] = QF[qf_idx(ld, ks, nb, i)];
         load_soa<NA>(As, ld, nb, in.an);
     };
 #pragma unroll
@@ -1596,7 +1604,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     };
     const size_t sQ = sizeof(T) * (size_t)NQ * ld, sA = sizeof(T) * (size_t)NA * ld;
     size_t oX = take(sQ), oW = take(sQ), oF = take(sQ), oA = take(sA), oAs = take(sA), oAt = take(sA);
-    size_t oQF = take(sizeof(T) * (size_t)MFa * NQ * ld);
+    size_t oQF = take(sizeof(T) * (size_t)MFa * NQP * ld);
     size_t oXs = take(s->opt.positivity == 1 ? sQ : 8);   // MOOD: the saved start-of-step state
     size_t oIm = take(s->opt.time_integration == 1 ? 4 * sQ : 8);   // IMEX: G, stage state, P3, F
     size_t oPart = take(sizeof(T) * (size_t)s->maxblocks);
