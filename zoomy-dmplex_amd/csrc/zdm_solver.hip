@@ -167,6 +167,10 @@ struct DevMesh {
     const T* fnhx;    // N / area
     const T* fnhy;
     const int* fbc;   // model BC index of boundary faces, -1 = none
+    // k_flux's packed copies: ctopo[2 (k ld + c)] = cface, [.. + 1] = cnbr | cmeta << 28
+    // (-1 without a neighbour); fgeo[4 f] = (fnhx, fnhy, farea, 0)
+    const int* ctopo;
+    const T* fgeo;
     const T* lsqw;    // [MAXF][DIM][ld] least-squares weights
     // ensemble of nmem member meshes in one launch (nmem = 0: a single mesh): each member owns
     // whole workgroups, blocks [blk_first[m], blk_first[m+1]) cover its cells
@@ -681,17 +685,20 @@ __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params&
     int fk[MAXF], nk[MAXF], mk[MAXF];
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) {
-        fk[k] = m.cface[(size_t)k * ld + c];
-        nk[k] = m.cnbr[(size_t)k * ld + c];
-        mk[k] = m.cmeta[(size_t)k * ld + c];
+        const size_t e = 2 * ((size_t)k * ld + c);
+        const int nm = m.ctopo[e + 1];
+        fk[k] = m.ctopo[e];
+        nk[k] = nm >= 0 ? (nm & 0x0FFFFFFF) : -1;
+        mk[k] = nm >= 0 ? (nm >> 28) : 0;
     }
     auto load = [&](int k, FaceIn& in) {
         const int f = fk[k] >= 0 ? fk[k] : 0;
         const int nb = nk[k] >= 0 ? nk[k] : c;
         const int ks = nk[k] >= 0 ? (mk[k] & 3) : 0;
-        in.area = m.farea[f];
-        in.nh[0] = m.fnhx[f];
-        in.nh[1] = DIM > 1 ? m.fnhy[f] : (T)0;
+        const T* g = m.fgeo + 4 * (size_t)f;
+        in.nh[0] = g[0];
+        in.nh[1] = DIM > 1 ? g[1] : (T)0;
+        in.area = g[2];
         in.nh[2] = 0;
         in.xf[0] = m.fcx[f];
         in.xf[1] = DIM > 1 ? m.fcy[f] : (T)0;
@@ -1521,6 +1528,12 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     std::vector<T> fNx(nf), fNy(nf), fcx(nf), fcy(nf), farea(nf), fnhx(nf), fnhy(nf);
     std::vector<int> fbc(nf);
     std::vector<T> lsqw;
+    if (nc >= (1 << 28)) {
+        delete s;
+        return fail("mesh too large: the packed neighbour index holds fewer than 2^28 cells");
+    }
+    std::vector<int> ctopo((size_t)2 * MFa * ld, -1);
+    std::vector<T> fgeo((size_t)4 * std::max(nf, 1), 0.0);
     for (int f = 0; f < nf; ++f) {
         const double* N = mesh->face_normal + 3 * (size_t)f;
         fNx[f] = N[0];
@@ -1534,6 +1547,9 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
         fnhx[f] = N[0] / area;
         fnhy[f] = DIM > 1 ? N[1] / area : 0;
         fbc[f] = mesh->face_cells[2 * f + 1] >= 0 ? -1 : face_bc[f];
+        fgeo[4 * (size_t)f] = fnhx[f];
+        fgeo[4 * (size_t)f + 1] = fnhy[f];
+        fgeo[4 * (size_t)f + 2] = area;
     }
     for (int c = 0; c < nc; ++c) {
         vol[c] = mesh->cell_volume[c];
@@ -1557,6 +1573,8 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
                 meta |= ks;
             }
             cmeta[(size_t)k * ld + c] = meta;
+            ctopo[2 * ((size_t)k * ld + c)] = f;
+            ctopo[2 * ((size_t)k * ld + c) + 1] = nb >= 0 ? (nb | (meta << 28)) : -1;
             const double* xf = mesh->face_centroid + 3 * (size_t)f;
             T d2 = 0.0;
             for (int d = 0; d < DIM; ++d) {
@@ -1619,6 +1637,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     size_t oF7[7];
     for (int i = 0; i < 7; ++i) oF7[i] = take(sizeof(T) * (size_t)std::max(nf, 1));
     size_t oFbc = take(sizeof(int) * (size_t)std::max(nf, 1));
+    size_t oCtopo = take(sizeof(int) * ctopo.size()), oFgeo = take(sizeof(T) * fgeo.size());
     size_t oLsq = take(sizeof(T) * (lsqw.empty() ? 1 : lsqw.size()));
     s->arena_bytes = off;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1659,6 +1678,8 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     const std::vector<T>* fa[7] = {&fNx, &fNy, &fcx, &fcy, &farea, &fnhx, &fnhy};
     for (int i = 0; i < 7 && nf > 0; ++i) e = e != hipSuccess ? e : up(oF7[i], fa[i]->data(), sizeof(T) * nf);
     if (nf > 0) e = e != hipSuccess ? e : up(oFbc, fbc.data(), sizeof(int) * nf);
+    e = e != hipSuccess ? e : up(oCtopo, ctopo.data(), sizeof(int) * ctopo.size());
+    e = e != hipSuccess ? e : up(oFgeo, fgeo.data(), sizeof(T) * fgeo.size());
     if (!lsqw.empty()) e = e != hipSuccess ? e : up(oLsq, lsqw.data(), sizeof(T) * lsqw.size());
     if (nmem > 0) {
         e = e != hipSuccess ? e : up(oBm, blk_member.data(), sizeof(int) * blk_member.size());
@@ -1690,6 +1711,8 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     d.fnhx = (T*)(a + oF7[5]);
     d.fnhy = (T*)(a + oF7[6]);
     d.fbc = (int*)(a + oFbc);
+    d.ctopo = (int*)(a + oCtopo);
+    d.fgeo = (T*)(a + oFgeo);
     d.lsqw = (T*)(a + oLsq);
     d.blk_member = (int*)(a + oBm);
     d.blk_first = (int*)(a + oBf);
