@@ -853,8 +853,17 @@ __device__ __forceinline__ void newton_cell(const Params& P, T dt, T* q, const T
         if (std::sqrt(rn) < 1e-8) break;
         auto dSdQ = M::source_jacobian_wrt_variables(q, a, P.p);
         auto dSdA = M::source_jacobian_wrt_aux_variables(q, a, P.p);
+        // The aux chain term dS/dA dA/dq only where dS/dA has a nonzero entry: for a model
+        // whose dS/dA is identically zero the test folds at compile time and the finite-
+        // difference aux Jacobian is not formed (with a finite dA/dq the term adds a signed
+        // zero, which leaves J unchanged)
+        bool aux_term = false;
+#pragma unroll
+        for (int i = 0; i < NQ * NA; ++i) aux_term |= dSdA[i] != (T)0;
         T Jx[NA * NQ];
-        {
+#pragma unroll
+        for (int i = 0; i < NA * NQ; ++i) Jx[i] = 0;
+        if (aux_term) {
             auto a0 = M::update_aux_variables(q, a, P.p, 0.0, X0);
 #pragma unroll
             for (int j = 0; j < NQ; ++j) {
@@ -874,8 +883,10 @@ __device__ __forceinline__ void newton_cell(const Params& P, T dt, T* q, const T
 #pragma unroll
             for (int j = 0; j < NQ; ++j) {
                 T term = dSdQ[i * NQ + j];
+                if (aux_term) {
 #pragma unroll
-                for (int l = 0; l < NA; ++l) term += dSdA[i * NA + l] * Jx[l * NQ + j];
+                    for (int l = 0; l < NA; ++l) term += dSdA[i * NA + l] * Jx[l * NQ + j];
+                }
                 J[i * NQ + j] = (i == j ? (T)1.0 : (T)0.0) - dt * term;
             }
 #pragma unroll
