@@ -798,6 +798,9 @@ __device__ __forceinline__ bool gepp_reg(T* A, T* b, T* x) {
             }
         }
         if (mv < 1e-14) return false;
+        // the selects of the swap run only when a lane of the wavefront pivots off the diagonal
+        // (a diagonally dominant I - dt dS/dq rarely does)
+        if (__any(mr != k))
 #pragma unroll
         for (int i = k + 1; i < NQ; ++i) {
             const bool sw = (i == mr);
