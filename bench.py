@@ -46,6 +46,13 @@ def flux_pass_bytes_per_cell(n_q, faces_per_cell):
     return 48 * n_q + 48 + 40 * faces_per_cell
 
 
+def flux_launch_bytes_per_cell(n_q, faces_per_cell, stages=2):
+    """Mean algorithmic HBM bytes per cell of one k_flux launch of a step: s - 1 stage launches
+    of a flux pass and the final one, which also reads Q^n for the SSP combination (the step's
+    "8 n (Q^n in the last stage)" term): 48 n + 48 + 40 F + 8 n / s."""
+    return flux_pass_bytes_per_cell(n_q, faces_per_cell) + 8.0 * n_q / stages
+
+
 def step_bytes_per_cell(n_q, faces_per_cell, stages=2):
     """Algorithmic HBM bytes of one full step per cell:
     s (88 n + 96 + 56 F) + 8 n (Q^n in the last stage) + 16 n (source) + 8 n + 8 (CFL)."""
@@ -53,8 +60,9 @@ def step_bytes_per_cell(n_q, faces_per_cell, stages=2):
 
 
 def pmc_traffic(kernel, cells):
-    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE profile of the
-    same workload (profiles/k_flux_c2_2M_pmc.md, copy-kernel calibrated), or None."""
+    """HBM bytes per launch of `kernel` (k_flux: the mean of its stage and final launches) from
+    the committed FETCH_SIZE / WRITE_SIZE profile of the same workload
+    (profiles/k_flux_c2_2M_pmc.md, copy-kernel calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "k_flux_c2_2M_pmc.md")
     try:
         for line in open(path):
@@ -592,7 +600,7 @@ def main():
     ncells = mesh.ncells
     F = mesh.nfaces / ncells
     value = ncells * args.steps / elapsed / 1e6
-    flux_bytes = flux_pass_bytes_per_cell(info.n_q, F) * ncells
+    flux_bytes = flux_launch_bytes_per_cell(info.n_q, F) * ncells
     achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
     out = {
         "metric": "Mcell-steps/s (SME L2, FP64)",
@@ -626,10 +634,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": pmc_traffic("k_flux<3, 2>", ncells),
-            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of the stage launch, copy-kernel "
-                              "calibrated (profiles/k_flux_c2_2M_pmc.md)",
-            "bytes_per_cell": flux_pass_bytes_per_cell(info.n_q, F),
+            "traffic": pmc_traffic("k_flux", ncells),
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, mean of the stage and final "
+                              "launches, copy-kernel calibrated (profiles/k_flux_c2_2M_pmc.md)",
+            "bytes_per_cell": round(flux_launch_bytes_per_cell(info.n_q, F), 2),
             "launches": flux_n,
         },
     }
