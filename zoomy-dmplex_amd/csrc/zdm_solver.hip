@@ -1104,8 +1104,22 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(Ctrl* ctrl, const T* __restr
     if ((what == 2 || what == 3) && !ctrl->active) return;
     const int b0 = blk_first ? blk_first[blockIdx.x] : 0;
     const int b1 = blk_first ? blk_first[blockIdx.x + 1] : np;
-    T v = DBL_MAX;
-    for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) v = partials[i] < v ? partials[i] : v;
+    // four independent minima per thread keep four loads in flight (the minimum is exact in
+    // any order)
+    T v0 = DBL_MAX, v1 = DBL_MAX, v2 = DBL_MAX, v3 = DBL_MAX;
+    const int st = blockDim.x;
+    int i = b0 + threadIdx.x;
+    for (; i + 3 * st < b1; i += 4 * st) {
+        const T p0 = partials[i], p1 = partials[i + st], p2 = partials[i + 2 * st], p3 = partials[i + 3 * st];
+        v0 = p0 < v0 ? p0 : v0;
+        v1 = p1 < v1 ? p1 : v1;
+        v2 = p2 < v2 ? p2 : v2;
+        v3 = p3 < v3 ? p3 : v3;
+    }
+    for (; i < b1; i += st) v0 = partials[i] < v0 ? partials[i] : v0;
+    v0 = v1 < v0 ? v1 : v0;
+    v2 = v3 < v2 ? v3 : v2;
+    T v = v2 < v0 ? v2 : v0;
     v = block_min(v);
     if (threadIdx.x != 0) return;
     const bool fin = !(!isfinite(v) || v >= 0.5 * DBL_MAX);
