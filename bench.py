@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "zoomy-dmplex_amd")
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, GB/s
+TIMING_STRIDE = 10      # C2: per-kernel HIP events on every 10th timed step
 
 
 def load_package():
@@ -546,6 +547,10 @@ def main():
     ap.add_argument("--order", default="rcm", choices=["hilbert", "rcm", "none"],
                     help="cell renumbering of the mesh service")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="C2: keep the default launch shape (256 x 1) instead of the autotuned one")
+    ap.add_argument("--launch", default=None, metavar="BSxCPT",
+                    help="C2: this launch shape (workgroup size x cells per thread), no autotuning")
     ap.add_argument("--time-integration", default="splitting", choices=["splitting", "imex", "implicit"],
                     help="c2 strategy: splitting (the headline), imex (ARK2), implicit (BDF2 Newton-Krylov)")
     ap.add_argument("--no-limiter", dest="limiter", action="store_false",
@@ -581,8 +586,19 @@ def main():
     s.initialize()
     s.step(args.warmup)
     s.synchronize()
+    # launch shape of the transport kernels by measurement (the autotuner restores the state)
+    shape, tune = (256, 1), {}
+    if args.launch:
+        shape = tuple(int(v) for v in args.launch.lower().split("x"))
+        s.set_launch(*shape)
+    elif args.autotune:
+        bs, cpt, tune = s.autotune(3)
+        shape = (bs, cpt)
+    s.synchronize()
 
-    s.enable_timing(True)
+    # kernel events on every TIMING_STRIDE-th step of the timed region: live per-kernel times
+    # without a marker between the kernels of every step
+    s.enable_timing(TIMING_STRIDE)
     s.synchronize()
     t0 = time.perf_counter()
     s.step(args.steps)
@@ -625,6 +641,9 @@ def main():
             "step_bytes_per_cell": round(step_bytes_per_cell(info.n_q, F), 1),
             "kernel_ms": {"prep": round(prep_ms, 4), "flux": round(flux_ms, 4),
                           "post": round(post_ms, 4)},
+            "kernel_ms_sample": "HIP events on every %d-th timed step (%d flux launches)" % (TIMING_STRIDE, flux_n),
+            "launch": {"block_size": shape[0], "cells_per_thread": shape[1],
+                       "autotune_ms_per_step": {"%dx%d" % k: round(v, 4) for k, v in tune.items()}},
             "sim_time": t_sim,
         },
         "roofline": {

@@ -135,8 +135,9 @@ int zdm_state_upload_device(zdm_solver* s, const double* dX, const double* dA);
 int zdm_state_download_device(zdm_solver* s, double* dX, double* dA);
 
 /* Per-kernel HIP-event timing of zdm_step launches while enabled (ms, averaged per launch;
- * events are resolved without synchronising inside zdm_step).  Kernel ids:
- * 0 = prep (gradient/bounds/reconstruction), 1 = flux + stage update, 2 = post-step. */
+ * events are resolved without synchronising inside zdm_step).  on = 0 off, 1 every step, k > 1
+ * every k-th step of zdm_step (a sample that leaves the other steps without event markers).
+ * Kernel ids: 0 = prep (gradient/bounds/reconstruction), 1 = flux + stage update, 2 = post-step. */
 int zdm_enable_timing(zdm_solver* s, int on);
 int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches);
 

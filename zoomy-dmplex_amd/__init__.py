@@ -563,6 +563,7 @@ class Solver:
         return d.value
 
     def enable_timing(self, on=True):
+        """HIP-event kernel timing: False off, True every step, k > 1 every k-th step."""
         self._check(self.lib.zdm_enable_timing(self._h, int(on)))
 
     def kernel_time(self, kernel):

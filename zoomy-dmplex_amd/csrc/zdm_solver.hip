@@ -1222,7 +1222,9 @@ struct zdm_solver {
     T* Im;                            // IMEX: stage right-hand side, stage state, running sums P3, F
     double cfl_in;                    // solver.cfl as given (opt.cfl: capped for zhang_shu)
     Ctrl *ctrl, *ctrl_rhs;
-    bool timing;
+    bool timing;                      // record kernel events on the step being enqueued
+    int timing_stride;                // zdm_enable_timing: events on every timing_stride-th step
+    long long timing_step;
     std::vector<hipEvent_t> evpool;   // recorded in order; pending until resolved
     std::vector<int> evkind;          // kernel id that ends at event i (-1 = segment start)
     size_t evused;
@@ -1747,6 +1749,8 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     d.cell_off = (int*)(a + oCo);
     d.nmem = nmem;
     s->timing = false;
+    s->timing_stride = 0;
+    s->timing_step = 0;
     s->evused = 0;
     for (int i = 0; i < 3; ++i) {
         s->kt_ms[i] = 0;
@@ -1991,15 +1995,18 @@ static void enqueue_step(zdm_solver* s) {
 int zdm_step(zdm_solver* s, int n) {
     if (!s) return fail("null solver");
     for (int step = 0; step < n; ++step) {
+        if (s->timing_stride > 0) s->timing = s->timing_step++ % s->timing_stride == 0;
         if (s->timing && s->evpool.size() > 4096 && s->evused + 64 > s->evpool.size())
             if (resolve_events(s)) return -1;
         enqueue_step(s);
         if (s->step_err) {
             s->step_err = 0;
+            s->timing = s->timing_stride > 0;
             return -1;
         }
         HIPCHK(hipGetLastError());
     }
+    s->timing = s->timing_stride > 0;
     return 0;
 }
 
@@ -2025,6 +2032,8 @@ int zdm_step_graph(zdm_solver* s, int n) {
 int zdm_enable_timing(zdm_solver* s, int on) {
     if (!s) return fail("null solver");
     if (resolve_events(s)) return -1;
+    s->timing_stride = on == 0 ? 0 : on < 0 ? 1 : on;
+    s->timing_step = 0;
     s->timing = on != 0;
     for (int i = 0; i < 3; ++i) {
         s->kt_ms[i] = 0;
