@@ -69,3 +69,27 @@ def test_ensemble_shapes_bitwise_equal_and_autotune_restores_state():
     assert np.array_equal(before[0][0], after[0][0]) and before[1] == after[1]
     e.step(3)
     e.close()
+
+
+def test_kernel_timing_every_kth_step():
+    """zdm_enable_timing(s, k): events on every k-th step of zdm_step only (the per-kernel
+    averages come from that sample; the run itself is unchanged)."""
+    model = "sme1"
+    info = zdm.ModelInfo(model)
+    mesh = zdm.Mesh.tri_grid(20, 20, jitter=0.15, seed=6)
+    X = state(mesh, info.n_q, 6)
+    out = []
+    for stride in (0, 1, 3):
+        s = zdm.Solver(model, mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e9)
+        s.upload(X)
+        s.initialize()
+        s.enable_timing(stride)
+        s.step(9)
+        ms, n = s.kernel_time(1)
+        expect = {0: 0, 1: 2 * 9, 3: 2 * 3}[stride]   # two flux launches per SSP-RK2 step
+        assert n == expect, (stride, n)
+        assert (ms > 0) == (expect > 0)
+        s.enable_timing(False)
+        out.append(s.download()[0])
+        s.close()
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
