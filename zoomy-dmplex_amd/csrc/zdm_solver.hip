@@ -569,7 +569,22 @@ __global__ __launch_bounds__(BLOCK) void k_prep_alpha(DevMesh m, Params P, const
 }
 
 // ------------------------------------------------------------------------------ k_flux
-enum { MODE_RHS = 0, MODE_EULER = 1, MODE_STAGE = 2, MODE_FINAL = 3 };
+// ARK2 tableau of the IMEX step (imex.inc): Giraldo, Kelly and Constantinescu, SIAM J. Sci.
+// Comput. 35(5), 2013
+namespace ark2 {
+constexpr double R2 = 1.41421356237309504880;   // sqrt(2)
+constexpr double G = 1.0 - 1.0 / R2;            // gamma, the SDIRK diagonal
+constexpr double D = 1.0 / (2.0 * R2);
+constexpr double A21 = 2.0 * G;
+constexpr double A32 = (3.0 + 2.0 * R2) / 6.0;
+constexpr double A31 = 1.0 - A32;
+constexpr double AT21 = G, AT31 = D, AT32 = D;
+constexpr double B1 = D, B2 = D, B3 = G;
+constexpr double C2 = 2.0 * G;                   // c = (0, 2 g, 1)
+}  // namespace ark2
+
+// MODE_IMEX3: the last term of the IMEX step, X+ = F + dt b3 G3 with F = X0 (imex.inc)
+enum { MODE_RHS = 0, MODE_EULER = 1, MODE_STAGE = 2, MODE_FINAL = 3, MODE_IMEX3 = 4 };
 
 // One face slot of the face loop of ComputeFluxes (TransportStep.hpp:383-565) for a slot with
 // f >= 0 and area > 1e-15: the face aux, flux and fluctuations of the (own qo, neighbour qn with
@@ -733,12 +748,16 @@ __device__ __forceinline__ void stage_update(const Ctrl* ctrl, int c, int ld, co
         const T a = ctrl->a_stage;
 #pragma unroll
         for (int i = 0; i < NQ; ++i) Out[(size_t)i * ld + c] = In[(size_t)i * ld + c] + a * fc[i];
-    } else {
+    } else if constexpr (MODE == MODE_FINAL) {
         // VecAXPBYPCZ(X, (s-1)/s, dt/s, 1/s, W, F)
         const T al = ctrl->al, be = ctrl->be, ga = ctrl->ga;
 #pragma unroll
         for (int i = 0; i < NQ; ++i)
             Out[(size_t)i * ld + c] = al * In[(size_t)i * ld + c] + be * fc[i] + ga * X0[(size_t)i * ld + c];
+    } else {
+        const T dt = ctrl->dt;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) Out[(size_t)i * ld + c] = X0[(size_t)i * ld + c] + dt * (ark2::B3 * fc[i]);
     }
 }
 
@@ -1328,6 +1347,9 @@ void launch_flux_t(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T
             break;
         case MODE_STAGE:
             k_flux<MAXF, MODE_STAGE, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            break;
+        case MODE_IMEX3:
+            k_flux<MAXF, MODE_IMEX3, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
             break;
         default:
             k_flux<MAXF, MODE_FINAL, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
@@ -1940,9 +1962,8 @@ static void enqueue_imex_step(zdm_solver* s) {
     k_imex_stage3<<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->A, G, P3, F, Y, src);
     prep(s, s->ctrl, Y, s->A);
     record(s, 0);
-    flux(s, s->ctrl, MODE_RHS, 2, Y, G, s->A);
+    flux(s, s->ctrl, MODE_IMEX3, 2, Y, s->X, s->A, nullptr, nullptr, F);   // X = F + dt b3 G3
     record(s, 1);
-    k_imex_final<<<g, b, 0, s->stream>>>(s->dm, s->ctrl, s->X, F, G);
     if (s->opt.positivity == 1)
         post_mood(s);
     else
