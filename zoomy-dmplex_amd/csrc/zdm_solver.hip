@@ -553,6 +553,35 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
     }
 }
 
+// k_prep (Green-Gauss, the launch shape of k_prep) that also stores each face's limiter value
+// alpha in abuf[k ld + c]: the recording forward of the adjoint
+template <int MAXF, bool MC>
+__global__ __launch_bounds__(BLOCK, 3) void k_prep_rec(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
+                                                    const T* __restrict__ X, const T* __restrict__ Ag,
+                                                    T* __restrict__ As, T* __restrict__ QF, int use_limiters,
+                                                    T* __restrict__ abuf) {
+    const auto L = locate<MC>(m);
+    ctrl += L.m;
+    if (!ctrl->active) return;
+    const int ld = m.ld;
+    auto body = [&](int c) {
+        T ac[NA];
+        auto put = [&](int k, int i, T v) { QF[qf_idx(ld, k, c, i)] = v; };
+        recon_cell<MAXF, 1, decltype(put), 1>(m, P, X, Ag, c, use_limiters, ac, put, (T)0, 0, abuf);
+        store_soa<NA>(As, ld, c, ac);
+    };
+    if constexpr (MC) {
+        for (int j = 0; j < m.cpt; ++j) {
+            const int c = L.cell(j);
+            if (c >= m.ncells) return;
+            body(c);
+        }
+    } else {
+        if (L.c >= m.ncells) return;
+        body(L.c);
+    }
+}
+
 // k_prep of one mesh (single-cell launch shape) with the limiter values stored (AMODE 1) or
 // taken from abuf (AMODE 2), for the matrix-free Jacobian of the implicit step
 template <int MAXF, int GRAD, int AMODE>
@@ -1363,6 +1392,25 @@ void launch_flux(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T* 
         launch_flux_t<MAXF, true>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
     else
         launch_flux_t<MAXF, false>(s, ctrl, mode, stage, In, Out, Ag, QF, As, X0);
+}
+
+// prep (Green-Gauss) with the limiter values stored in abuf: the adjoint's recording forward
+void prep_record(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* QF, T* As, T* abuf) {
+    dim3 g(s->nblocks), b(s->bs);
+    const int lim = s->opt.use_limiters;
+#define ZDM_PREC(MF)                                                                                      \
+    if (s->dm.cpt > 1)                                                                                   \
+        k_prep_rec<MF, true><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, abuf);     \
+    else                                                                                                 \
+        k_prep_rec<MF, false><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, abuf);
+    if (s->maxf <= 2) {
+        ZDM_PREC(2)
+    } else if (s->maxf == 3) {
+        ZDM_PREC(3)
+    } else {
+        ZDM_PREC(4)
+    }
+#undef ZDM_PREC
 }
 
 // face states / clamped aux into (QF, As): the solver's own buffers unless given
