@@ -260,8 +260,9 @@ __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
 
 // ------------------------------------------------------------------------------ k_prep
 // GRAD: 0 = none (PCM), 1 = Green-Gauss, 2 = least squares
-// 3 waves / SIMD: the hoisted neighbour loads need ~170 VGPRs; measured faster than the
-// 4-wave form with the loads issued inside the face loop.
+// 3 waves / SIMD: the hoisted neighbour loads need ~170 VGPRs (a 20-byte spill at the cap of
+// 168); measured faster than the 4-wave form with the loads issued inside the face loop, and
+// than 2 waves without the spill (0.188 against 0.209 ms at 2 M cells).
 // Clamp + aux of cell c (UpdateState), its gradient of the raw state (GRAD 1: Green-Gauss,
 // 2: least squares, 3: Green-Gauss with the eta-WB limiting of positivity "mood" / "zhang_shu"),
 // the neighbour bounds of the clamped states and the limited linear reconstruction at every face
