@@ -196,6 +196,21 @@ int zdm_vam_pmat(zdm_solver* s, double dt, int cap, int* rowptr, int* col, doubl
 int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val, const double* b,
                    double* x, const zdm_krylov_options* opts, int* its, int* reason);
 
+/* zdm_bsr2_gmres for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same kernels and the same results,
+ * with rowptr, col, val, b and x (initial guess in, solution out) as DEVICE arrays and the work
+ * ordered on `stream` (a hipStream_t; NULL = the null stream).  It calls no hipMalloc / hipFree:
+ * its workspace (the point-block Jacobi inverse, the Krylov basis, the reduction partials and
+ * the Hessenberg state) lives in a 64 MiB device array the library holds on each GPU until the
+ * process ends, which bounds the size: about 560 * nb + 270 000 bytes for restart 30; larger
+ * systems are rejected with an error.  Every call on one GPU shares that array (the contents
+ * of the last call remain; each call clears what it needs), so calls that overlap on one GPU
+ * race.  The structure is checked on the device (row pointers from 0, non-decreasing; columns
+ * in [0, nb)).  Synchronises `stream` once per restart cycle.  No counterpart in the
+ * reference: it exists for the callers named above, which cannot allocate. */
+int zdm_bsr2_gmres_noalloc(int nb, const int* rowptr, const int* col, const double* val, const double* b,
+                           double* x, const zdm_krylov_options* opts, void* stream, int* its, int* reason);
+
 /* ---- Discrete adjoint of the time step (the differentiation form).  A calibration problem
  * of nsteps MUSCLSolver steps from (X0, A0) with the loss
  *     L = sum_{k>=1} sum_g (h(gauges[g]) after step k*obs_every - obs[(k-1)*ngauge + g])^2,

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -1523,6 +1524,80 @@ int read_ctrls(zdm_solver* s, std::vector<Ctrl>& cs) {
 
 }  // namespace
 
+// Scratch of zdm_bsr2_gmres_noalloc: device memory the library holds on every GPU it runs on
+// until the process ends, so that entry point allocates nothing.  Host code never uses the
+// name `scratch` as an address (on the host it is only a stand-in for the device variable):
+// scratch_on_current_device() resolves this GPU's copy once, and only that pointer reaches
+// kernels and copies.
+constexpr size_t ZDM_SCRATCH_BYTES = size_t(64) << 20;   // 64 MiB per GPU
+static __device__ __align__(256) double scratch[ZDM_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int ZDM_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[ZDM_MAX_DEVICES] = {};
+
+// this GPU's address of `scratch` (resolved on the first call with the device current)
+int scratch_on_current_device(char** out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= ZDM_MAX_DEVICES) return fail("scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail("scratch: no device address");
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// flag[0] = 1 unless rowptr[0] == 0 and rowptr never decreases
+__global__ void k_bsr2_check_rows(int nb, const int* __restrict__ rowptr, int* __restrict__ flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nb) return;
+    const int lo = rowptr[c], hi = rowptr[c + 1];
+    if (hi < lo || (c == 0 && lo != 0)) flag[0] = 1;
+}
+
+// flag[0] = 1 if a block column lies outside [0, nb)
+__global__ void k_bsr2_check_cols(int nb, int nnzb, const int* __restrict__ col, int* __restrict__ flag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnzb) return;
+    const int j = col[k];
+    if (j < 0 || j >= nb) flag[0] = 1;
+}
+
+// GMRES + point-block Jacobi on a 2x2-block CSR matrix whose arrays, b and x are on the device;
+// Dinv (4 per block row) and kmem (gmres_bytes) are the caller's workspace.  The workspace is
+// cleared by a fill kernel, so every call starts from the same state.
+int bsr2_gmres_on_device(const Bsr2& A, const T* b, T* x, T* Dinv, char* kmem, const zdm_krylov_options& o,
+                         hipStream_t st, const char* who, int* its, int* reason) {
+    const size_t n = 2 * (size_t)A.nb;
+    const size_t kd = gmres_bytes(o.restart, n) / sizeof(T);
+    k_fill<<<(unsigned)((kd + 255) / 256), 256, 0, st>>>(reinterpret_cast<T*>(kmem), kd, 0.0);
+    const unsigned cg = (unsigned)((A.nb + 255) / 256);
+    k_pbj_from_bsr<<<cg, 256, 0, st>>>(A, Dinv);
+    Gmres g;
+    gmres_bind(g, kmem, o.restart, n);
+    g.max_it = o.max_it;
+    g.rtol = o.rtol;
+    g.atol = o.atol;
+    g.dtol = o.dtol;
+    auto op = [&](const T* xx, T* y, const int* h) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 0, h); };
+    auto op_pc = [&](const T* xx, T* y, const int* h) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 1, h); };
+    auto pc = [&](const T* bb, const T* t, T* z) { k_pbj_residual<<<cg, 256, 0, st>>>(A.nb, Dinv, bb, t, z); };
+    const int r = gmres_solve(g, st, op_pc, op, pc, b, x);
+    if (r == -100 || hipGetLastError() != hipSuccess) return fail(std::string(who) + ": device error in the solve");
+    if (its) *its = g.last_its;
+    if (reason) *reason = g.last_reason;
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* zdm_last_error(void) { return g_err.c_str(); }
@@ -2293,7 +2368,6 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
         if (e == hipSuccess && nnzb) e = hipMemcpyAsync(mem + oV, val, sizeof(T) * 4 * nnzb, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(mem + oB, b, sizeof(T) * n, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(mem + oX, x, sizeof(T) * n, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipMemsetAsync(kmem, 0, gmres_bytes(o.restart, n), st);
         if (e != hipSuccess) {
             rc = fail(std::string("zdm_bsr2_gmres: upload failed: ") + hipGetErrorString(e));
         } else {
@@ -2302,28 +2376,11 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
             A.rowptr = (const int*)(mem + oR);
             A.col = (const int*)(mem + oC);
             A.val = (const T*)(mem + oV);
-            T* Dinv = (T*)(mem + oD);
-            const unsigned cg = (unsigned)((nb + 255) / 256);
-            k_pbj_from_bsr<<<cg, 256, 0, st>>>(A, Dinv);
-            Gmres g;
-            gmres_bind(g, kmem, o.restart, n);
-            g.max_it = o.max_it;
-            g.rtol = o.rtol;
-            g.atol = o.atol;
-            g.dtol = o.dtol;
-            auto op = [&](const T* xx, T* y, const int* h) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 0, h); };
-            auto op_pc = [&](const T* xx, T* y, const int* h) { k_bsr2_spmv<<<cg, KBLOCK, 0, st>>>(A, Dinv, xx, y, 1, h); };
-            auto pc = [&](const T* bb, const T* t, T* z) { k_pbj_residual<<<cg, 256, 0, st>>>(nb, Dinv, bb, t, z); };
-            const int r = gmres_solve(g, st, op_pc, op, pc, (const T*)(mem + oB), (T*)(mem + oX));
-            if (r == -100 || hipGetLastError() != hipSuccess) {
-                rc = fail("zdm_bsr2_gmres: device error in the solve");
-            } else if (hipMemcpyAsync(x, mem + oX, sizeof(T) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                       hipStreamSynchronize(st) != hipSuccess) {
+            rc = bsr2_gmres_on_device(A, (const T*)(mem + oB), (T*)(mem + oX), (T*)(mem + oD), kmem, o, st,
+                                      "zdm_bsr2_gmres", its, reason);
+            if (rc == 0 && (hipMemcpyAsync(x, mem + oX, sizeof(T) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                            hipStreamSynchronize(st) != hipSuccess))
                 rc = fail("zdm_bsr2_gmres: download failed");
-            } else {
-                if (its) *its = g.last_its;
-                if (reason) *reason = g.last_reason;
-            }
         }
     }
     (void)hipStreamSynchronize(st);
@@ -2331,6 +2388,52 @@ int zdm_bsr2_gmres(int nb, const int* rowptr, const int* col, const double* val,
     if (kmem) (void)hipFree(kmem);
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+int zdm_bsr2_gmres_noalloc(int nb, const int* rowptr, const int* col, const double* val, const double* b,
+                           double* x, const zdm_krylov_options* opts, void* stream, int* its, int* reason) {
+    if (nb < 1 || !rowptr || !col || !val || !b || !x) return fail("zdm_bsr2_gmres_noalloc: bad arguments");
+    zdm_krylov_options o = opts ? *opts : krylov_defaults();
+    if (check_krylov(o)) return -1;
+    const size_t n = 2 * (size_t)nb;
+    // scratch: [flags (256 B)] [Dinv, 4 per block] [GMRES workspace]
+    const size_t oD = 256, oK = align_up(oD + sizeof(T) * 4 * (size_t)nb, 256);
+    const size_t kbytes = gmres_bytes(o.restart, n), need = oK + kbytes;
+    if (need > sizeof(scratch))
+        return fail("zdm_bsr2_gmres_noalloc: " + std::to_string(nb) + " block rows with restart " +
+                    std::to_string(o.restart) + " need " + std::to_string(need) +
+                    " bytes of scratch, more than its " + std::to_string(sizeof(scratch)) + " bytes");
+    char* sp = nullptr;
+    if (scratch_on_current_device(&sp)) return -1;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    int* flag = reinterpret_cast<int*>(sp);
+    // the structure is checked on the device (the arrays never come to the host): first the row
+    // pointers, and only when they start at 0 and never decrease, the columns they span
+    k_fill<<<1, 64, 0, st>>>(reinterpret_cast<T*>(sp), 256 / sizeof(T), 0.0);
+    const unsigned cg = (unsigned)((nb + 255) / 256);
+    k_bsr2_check_rows<<<cg, 256, 0, st>>>(nb, rowptr, flag);
+    HIPCHK(hipGetLastError());
+    int hflag[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(hflag, flag, sizeof(hflag), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hflag[0]) return fail("zdm_bsr2_gmres_noalloc: row pointers must start at 0 and not decrease");
+    int nnzb = 0;
+    HIPCHK(hipMemcpyAsync(&nnzb, rowptr + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (nnzb > 0) {
+        k_bsr2_check_cols<<<(unsigned)((nnzb + 255) / 256), 256, 0, st>>>(nb, nnzb, col, flag + 1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hflag, flag, sizeof(hflag), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (hflag[1]) return fail("zdm_bsr2_gmres_noalloc: block column out of range");
+    }
+    Bsr2 A;
+    A.nb = nb;
+    A.rowptr = rowptr;
+    A.col = col;
+    A.val = val;
+    return bsr2_gmres_on_device(A, b, x, reinterpret_cast<T*>(sp + oD), sp + oK, o, st,
+                                "zdm_bsr2_gmres_noalloc", its, reason);
 }
 
 #ifdef ZOOMY_MODEL_IS_VAM
