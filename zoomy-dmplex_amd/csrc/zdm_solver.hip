@@ -172,6 +172,10 @@ struct DevMesh {
     // (-1 without a neighbour); fgeo[4 f] = (fnhx, fnhy, farea, 0)
     const int* ctopo;
     const T* fgeo;
+    // per face slot of each cell, two 16-byte records (sgeo_idx): the area-weighted normal
+    // signed outward from the cell (-N where the cell is support[1]) and the face centroid
+    // minus the cell centroid; zero for empty slots
+    const double2* sgeo;
     const T* lsqw;    // [MAXF][DIM][ld] least-squares weights
     // ensemble of nmem member meshes in one launch (nmem = 0: a single mesh): each member owns
     // whole workgroups, blocks [blk_first[m], blk_first[m+1]) cover its cells
@@ -183,6 +187,10 @@ struct DevMesh {
     int cpt;          // cells per thread of the transport kernels: thread t of a workgroup takes
                       // cells first + t + j * blockDim, j < cpt
 };
+
+__host__ __device__ __forceinline__ size_t sgeo_idx(int ld, int k, int part, int c) {
+    return ((size_t)k * 2 + part) * ld + c;
+}
 
 __device__ __forceinline__ int logical_block() {
     // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give each XCD a
@@ -296,24 +304,26 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
     } else {
         // all independent loads of the cell first (indices, then neighbour states and face
         // geometry through in-bounds clamped indices), so they are in flight together
-        int fk[MAXF], nbk[MAXF], mk[MAXF];
+        // face geometry from the cell's own slot records (coalesced): Nk = the area-weighted
+        // normal signed outward from this cell, dxk = face centroid - cell centroid
+        int fk[MAXF], nbk[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             fk[k] = m.cface[(size_t)k * ld + c];
             nbk[k] = m.cnbr[(size_t)k * ld + c];
-            mk[k] = m.cmeta[(size_t)k * ld + c];
         }
-        T qn[MAXF][NQ], an[MAXF][NA], Nk[MAXF][2], xfk[MAXF][2];
+        T qn[MAXF][NQ], an[MAXF][NA], Nk[MAXF][2], dxk[MAXF][2];
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             const int nbs = nbk[k] >= 0 ? nbk[k] : c;
-            const int fs = fk[k] >= 0 ? fk[k] : 0;
             load_soa<NQ>(X, ld, nbs, qn[k]);
             load_soa<NA>(Ag, ld, nbs, an[k]);
-            Nk[k][0] = m.fNx[fs];
-            Nk[k][1] = DIM > 1 ? m.fNy[fs] : (T)0;
-            xfk[k][0] = m.fcx[fs];
-            xfk[k][1] = DIM > 1 ? m.fcy[fs] : (T)0;
+            const double2 ns = m.sgeo[sgeo_idx(ld, k, 0, c)];
+            const double2 dx = m.sgeo[sgeo_idx(ld, k, 1, c)];
+            Nk[k][0] = ns.x;
+            Nk[k][1] = ns.y;
+            dxk[k][0] = dx.x;
+            dxk[k][1] = dx.y;
         }
         T lo[NQ], hi[NQ], g[NQ * DIM];
 #pragma unroll
@@ -325,7 +335,6 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
         }
 #pragma unroll
         for (int i = 0; i < NQ * DIM; ++i) g[i] = 0;
-        const T xc[2] = {m.cx[c], DIM > 1 ? m.cy[c] : (T)0};
 
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
@@ -342,18 +351,12 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
                     if (w > hi[i]) hi[i] = w;
                 }
                 if constexpr (GG) {
-                    const bool right = (mk[k] >> 2) & 1;
+                    // g -= fv N on the face's support[1] side is g += fv (-N) bit for bit
 #pragma unroll
                     for (int i = 0; i < NQ; ++i) {
                         const T fv = (T)0.5 * (q[i] + qn[k][i]);
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) {
-                            const T v = fv * N[d];
-                            if (right)
-                                g[i * DIM + d] -= v;
-                            else
-                                g[i * DIM + d] += v;
-                        }
+                        for (int d = 0; d < DIM; ++d) g[i * DIM + d] += fv * N[d];
                     }
                 } else {
 #pragma unroll
@@ -368,7 +371,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
                 const int bc = m.fbc[f];
                 if (bc >= 0) {
                     const T nh[3] = {m.fnhx[f], DIM > 1 ? m.fnhy[f] : (T)0, 0};
-                    const T xf[3] = {xfk[k][0], xfk[k][1], 0};
+                    const T xf[3] = {m.fcx[f], DIM > 1 ? m.fcy[f] : (T)0, 0};
                     auto qb = M::boundary_conditions(bc, q, nullptr, P.p, nh, xf, 0.0, 0.0);
 #pragma unroll
                     for (int i = 0; i < NQ; ++i)
@@ -403,7 +406,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
 #pragma unroll
             for (int k = 0; k < MAXF; ++k) {
                 if (fk[k] < 0) continue;
-                const T r[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
+                const T* r = dxk[k];
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
                     T delta = 0;
@@ -433,7 +436,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
 #pragma unroll
             for (int k = 0; k < MAXF; ++k) {
                 if (fk[k] < 0) continue;
-                const T r[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
+                const T* r = dxk[k];
                 T hf = h_bar;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) hf += sh[d] * r[d];
@@ -456,7 +459,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
 #pragma unroll
             for (int k = 0; k < MAXF; ++k) {
                 if (fk[k] < 0) continue;
-                const T dx[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
+                const T* dx = dxk[k];
                 T dot[NQ];
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
@@ -479,7 +482,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             if (fk[k] < 0) continue;
-            const T dx[2] = {xfk[k][0] - xc[0], DIM > 1 ? xfk[k][1] - xc[1] : (T)0};
+            const T* dx = dxk[k];
             T delta[NQ], fin[NQ];
 #pragma unroll
             for (int i = 0; i < NQ; ++i) {
@@ -500,15 +503,19 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
                     const T unl = qc[i] + delta[i];
                     const T dq = unl - qc[i];
                     if (std::fabs(dq) < 1e-12) continue;
+                    // min(1, num / dq) without the division where the quotient is >= 1
+                    // (rounding is monotone, so the result is the same bit for bit)
                     T a = 1.0;
                     if (dq > 0) {
+                        const T num = hi[i] - qc[i];
                         if (hi[i] > qc[i])
-                            a = std::min((T)1.0, (hi[i] - qc[i]) / dq);
+                            a = num < dq ? num / dq : (T)1.0;
                         else
                             a = 0.0;
                     } else {
+                        const T num = lo[i] - qc[i];
                         if (lo[i] < qc[i])
-                            a = std::min((T)1.0, (lo[i] - qc[i]) / dq);
+                            a = num > dq ? num / dq : (T)1.0;
                         else
                             a = 0.0;
                     }
@@ -1144,36 +1151,48 @@ __global__ __launch_bounds__(BLOCK) void k_post_record(DevMesh m, Params P, cons
 
 // what: 0 = standalone ComputeTimeStep (-> dt_query), 1 = first dt, 2 = end of a step,
 //       3 = end of a Chorin step (next dt = max(CFL, min_dt), ChorinVAM.hpp:103)
-// One workgroup per ensemble member (one in all for a single mesh): the member's partial minima
-// are those of its own workgroups.
-__global__ __launch_bounds__(BLOCK) void k_finalize(Ctrl* ctrl, const T* __restrict__ partials, int np,
-                                                    const int* __restrict__ blk_first, int what,
-                                                    int order, int s, T cfl, T min_dt, T t_end,
-                                                    T dt_max) {
-    ctrl += blockIdx.x;
-    if ((what == 2 || what == 3) && !ctrl->active) return;
-    const int b0 = blk_first ? blk_first[blockIdx.x] : 0;
-    const int b1 = blk_first ? blk_first[blockIdx.x + 1] : np;
-    // four independent minima per thread keep four loads in flight (the minimum is exact in
-    // any order)
-    T v0 = DBL_MAX, v1 = DBL_MAX, v2 = DBL_MAX, v3 = DBL_MAX;
+// The minimum of the partial minima in [b0, b1) with FACC independent minima per thread of a
+// FBLOCK-thread workgroup (a 2 M-cell step's ~31 000 partials are a few rounds of loads in
+// flight); the minimum is exact in any order.  Returns it in thread 0.
+constexpr int FBLOCK = 1024;   // k_finalize's workgroup
+constexpr int FACC = 16;
+
+__device__ __forceinline__ T range_min(const T* __restrict__ partials, int b0, int b1) {
+    T v[FACC];
+#pragma unroll
+    for (int a = 0; a < FACC; ++a) v[a] = DBL_MAX;
     const int st = blockDim.x;
     int i = b0 + threadIdx.x;
-    for (; i + 3 * st < b1; i += 4 * st) {
-        const T p0 = partials[i], p1 = partials[i + st], p2 = partials[i + 2 * st], p3 = partials[i + 3 * st];
-        v0 = p0 < v0 ? p0 : v0;
-        v1 = p1 < v1 ? p1 : v1;
-        v2 = p2 < v2 ? p2 : v2;
-        v3 = p3 < v3 ? p3 : v3;
+    for (; i + (FACC - 1) * st < b1; i += FACC * st) {
+        T p[FACC];
+#pragma unroll
+        for (int a = 0; a < FACC; ++a) p[a] = partials[i + a * st];
+#pragma unroll
+        for (int a = 0; a < FACC; ++a) v[a] = p[a] < v[a] ? p[a] : v[a];
     }
-    for (; i < b1; i += st) v0 = partials[i] < v0 ? partials[i] : v0;
-    v0 = v1 < v0 ? v1 : v0;
-    v2 = v3 < v2 ? v3 : v2;
-    T v = v2 < v0 ? v2 : v0;
-    v = block_min(v);
-    if (threadIdx.x != 0) return;
-    const bool fin = !(!isfinite(v) || v >= 0.5 * DBL_MAX);
-    const T dtc = fin ? cfl * v : dt_max;
+    for (; i < b1; i += st) v[0] = partials[i] < v[0] ? partials[i] : v[0];
+#pragma unroll
+    for (int a = 1; a < FACC; ++a) v[0] = v[a] < v[0] ? v[a] : v[0];
+    __shared__ T red[FBLOCK / 64];
+    T w = v[0];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T x = __shfl_xor(w, o, 64);
+        w = x < w ? x : w;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) w = red[k] < w ? red[k] : w;
+    return w;
+}
+
+// The CFL dt from the minimum ratio vmin and the control block of the next step
+// (MUSCLSolver.hpp:140-149); one thread.
+__device__ __forceinline__ void finish_step(Ctrl* ctrl, T vmin, int what, int order, int s, T cfl, T min_dt,
+                                            T t_end, T dt_max) {
+    const bool fin = !(!isfinite(vmin) || vmin >= 0.5 * DBL_MAX);
+    const T dtc = fin ? cfl * vmin : dt_max;
     if (what == 0) {
         ctrl->dt_query = dtc;
         ctrl->finite = fin;
@@ -1204,6 +1223,46 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(Ctrl* ctrl, const T* __restr
         ctrl->be = dt0 / s;
         ctrl->ga = 1. / s;
     }
+}
+
+// One workgroup per ensemble member: the member's partial minima are those of its own
+// workgroups.
+__global__ __launch_bounds__(FBLOCK) void k_finalize(Ctrl* ctrl, const T* __restrict__ partials, int np,
+                                                     const int* __restrict__ blk_first, int what,
+                                                     int order, int s, T cfl, T min_dt, T t_end,
+                                                     T dt_max) {
+    ctrl += blockIdx.x;
+    if ((what == 2 || what == 3) && !ctrl->active) return;
+    const int b0 = blk_first ? blk_first[blockIdx.x] : 0;
+    const int b1 = blk_first ? blk_first[blockIdx.x + 1] : np;
+    const T vmin = range_min(partials, b0, b1);
+    if (threadIdx.x == 0) finish_step(ctrl, vmin, what, order, s, cfl, min_dt, t_end, dt_max);
+}
+
+// A single run: NXCD workgroups, workgroup x over the partials the transport kernels' XCD x
+// wrote (logical_block gives each XCD one contiguous range, so the reads hit that XCD's L2).
+// Each writes its minimum to xmin[x]; the last to finish (an arrival counter, reset by it for
+// the next launch) takes the minimum of the NXCD values and updates the control block.  The
+// result does not depend on the arrival order.
+__global__ __launch_bounds__(FBLOCK) void k_finalize_xcd(Ctrl* ctrl, const T* __restrict__ partials, int np,
+                                                         T* xmin, unsigned* arrived, int what, int order,
+                                                         int s, T cfl, T min_dt, T t_end, T dt_max) {
+    if ((what == 2 || what == 3) && !ctrl->active) return;
+    const int x = blockIdx.x, q = np / NXCD, r = np % NXCD;
+    const int b0 = x * q + (x < r ? x : r), b1 = b0 + q + (x < r ? 1 : 0);
+    const T w = range_min(partials, b0, b1);
+    if (threadIdx.x != 0) return;
+    xmin[x] = w;
+    __threadfence();
+    if (atomicAdd(arrived, 1u) != (unsigned)(NXCD - 1)) return;
+    __threadfence();
+    T vmin = DBL_MAX;
+    for (int k = 0; k < NXCD; ++k) {
+        const T v = __hip_atomic_load(xmin + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vmin = v < vmin ? v : vmin;
+    }
+    *arrived = 0u;
+    finish_step(ctrl, vmin, what, order, s, cfl, min_dt, t_end, dt_max);
 }
 
 __global__ void k_aos_to_soa(const T* __restrict__ aos, T* __restrict__ soa, int n, int nc, int ld) {
@@ -1487,7 +1546,16 @@ void post(zdm_solver* s, int mode, T dt_override = 0) {
 }
 
 void finalize(zdm_solver* s, int what) {
-    k_finalize<<<s->nctrl, BLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, s->dm.nmem ? s->dm.blk_first : nullptr,
+    if (s->dm.nmem == 0) {
+        // xmin[NXCD] and the arrival counter follow the partials (zeroed with the arena)
+        T* xmin = s->partials + s->maxblocks;
+        k_finalize_xcd<<<NXCD, FBLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, xmin,
+                                                       reinterpret_cast<unsigned*>(xmin + NXCD), what,
+                                                       s->opt.order, s->opt.ssp_stages, s->opt.cfl,
+                                                       s->opt.min_dt, s->opt.t_end, (T)M::dt_max);
+        return;
+    }
+    k_finalize<<<s->nctrl, FBLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, s->dm.blk_first,
                                                  what, s->opt.order, s->opt.ssp_stages, s->opt.cfl,
                                                  s->opt.min_dt, s->opt.t_end, (T)M::dt_max);
 }
@@ -1710,6 +1778,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     }
     std::vector<int> ctopo((size_t)2 * MFa * ld, -1);
     std::vector<T> fgeo((size_t)4 * std::max(nf, 1), 0.0);
+    std::vector<double2> sgeo((size_t)MFa * 2 * ld, make_double2(0.0, 0.0));
     for (int f = 0; f < nf; ++f) {
         const double* N = mesh->face_normal + 3 * (size_t)f;
         fNx[f] = N[0];
@@ -1751,6 +1820,9 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
             cmeta[(size_t)k * ld + c] = meta;
             ctopo[2 * ((size_t)k * ld + c)] = f;
             ctopo[2 * ((size_t)k * ld + c) + 1] = nb >= 0 ? (nb | (meta << 28)) : -1;
+            const T sgn = (meta & 4) ? (T)-1 : (T)1;
+            sgeo[sgeo_idx(ld, k, 0, c)] = make_double2(sgn * fNx[f], sgn * fNy[f]);
+            sgeo[sgeo_idx(ld, k, 1, c)] = make_double2(fcx[f] - cx[c], fcy[f] - cy[c]);
             const double* xf = mesh->face_centroid + 3 * (size_t)f;
             T d2 = 0.0;
             for (int d = 0; d < DIM; ++d) {
@@ -1801,7 +1873,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     size_t oQF = take(sizeof(T) * (size_t)MFa * NQP * ld);
     size_t oXs = take(s->opt.positivity == 1 ? sQ : 8);   // MOOD: the saved start-of-step state
     size_t oIm = take(s->opt.time_integration == 1 ? 4 * sQ : 8);   // IMEX: G, stage state, P3, F
-    size_t oPart = take(sizeof(T) * (size_t)s->maxblocks);
+    size_t oPart = take(sizeof(T) * ((size_t)s->maxblocks + NXCD + 1));   // + k_finalize_xcd's minima, counter
     size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
     size_t oCtrl = take(sizeof(Ctrl) * s->nctrl), oCtrlR = take(sizeof(Ctrl) * s->nctrl);
     size_t oBm = take(sizeof(int) * std::max<size_t>(nmem > 0 ? (size_t)s->maxblocks : 0, 1)),
@@ -1814,6 +1886,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     for (int i = 0; i < 7; ++i) oF7[i] = take(sizeof(T) * (size_t)std::max(nf, 1));
     size_t oFbc = take(sizeof(int) * (size_t)std::max(nf, 1));
     size_t oCtopo = take(sizeof(int) * ctopo.size()), oFgeo = take(sizeof(T) * fgeo.size());
+    size_t oSgeo = take(sizeof(double2) * sgeo.size());
     size_t oLsq = take(sizeof(T) * (lsqw.empty() ? 1 : lsqw.size()));
     s->arena_bytes = off;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1856,6 +1929,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     if (nf > 0) e = e != hipSuccess ? e : up(oFbc, fbc.data(), sizeof(int) * nf);
     e = e != hipSuccess ? e : up(oCtopo, ctopo.data(), sizeof(int) * ctopo.size());
     e = e != hipSuccess ? e : up(oFgeo, fgeo.data(), sizeof(T) * fgeo.size());
+    e = e != hipSuccess ? e : up(oSgeo, sgeo.data(), sizeof(double2) * sgeo.size());
     if (!lsqw.empty()) e = e != hipSuccess ? e : up(oLsq, lsqw.data(), sizeof(T) * lsqw.size());
     if (nmem > 0) {
         e = e != hipSuccess ? e : up(oBm, blk_member.data(), sizeof(int) * blk_member.size());
@@ -1889,6 +1963,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     d.fbc = (int*)(a + oFbc);
     d.ctopo = (int*)(a + oCtopo);
     d.fgeo = (T*)(a + oFgeo);
+    d.sgeo = (const double2*)(a + oSgeo);
     d.lsqw = (T*)(a + oLsq);
     d.blk_member = (int*)(a + oBm);
     d.blk_first = (int*)(a + oBf);
