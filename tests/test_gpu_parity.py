@@ -110,6 +110,31 @@ def test_source_and_cfl_match_oracle(model):
     assert rel_err(Xs, Xc) <= 1e-14
 
 
+@pytest.mark.parametrize("model", ["sme1", "sme2", "sme1m"])
+@pytest.mark.parametrize("dt", [0.05, 10.0, 1e3])
+def test_source_newton_with_row_exchanges_matches_oracle(model, dt):
+    """Large source steps make |dt dS/dh| exceed the unit pivot of the depth row, so GEPP
+    exchanges rows in some cells (the device's exchange path) and not in others."""
+    info, mesh, X = make_case(model)
+    rng = np.random.default_rng(17)
+    X[:, 2:] = 0.3 * rng.standard_normal((mesh.ncells, info.n_q - 2))
+    s = zdm.Solver(model, mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e9)
+    A = np.zeros((mesh.ncells, info.n_aux))
+    if model == "sme1m":
+        A[:, 1] = 0.03
+    s.upload(X, A)
+    s.enforce_constraints()
+    Xe, Ae = s.download()
+    print("case", model, "source dt", dt, flush=True)
+    s.source_step(dt)
+    Xs, _ = s.download()
+    s.close()
+    Xc, Ac = Xe.copy(), Ae.copy()
+    oracle(model).source(s.params, dt, Xc, Ac)
+    assert np.all(np.isfinite(Xc))
+    assert np.array_equal(Xs, Xc), rel_err(Xs, Xc)
+
+
 def test_c1_mass_and_parity_200_steps():
     """C1 (SME-L0 dam break over a bump, 60x60 grid): 200 SSP-RK2 steps vs the oracle."""
     mesh = zdm.Mesh.tri_grid(60, 60)

@@ -889,6 +889,202 @@ __device__ __forceinline__ bool gepp_reg(T* A, T* b, T* x) {
     return true;
 }
 
+// The Newton residual R = q - qo - dt S(q, a) of SourceStep::SolveLocalCell (SourceStep.hpp:
+// 151-158), its squared norm returned
+__device__ __forceinline__ T newton_residual(const Params& P, T dt, const T* q, const T* qo, const T* a, T* r) {
+    const T X0[3] = {0, 0, 0};
+    auto S = M::source(q, a, P.p, (T)0, dt, X0);
+    T rn = 0.0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        r[i] = q[i] - qo[i] - dt * S[i];
+        rn += r[i] * r[i];
+    }
+    return rn;
+}
+
+// J = I - dt (dS/dq + dS/da da/dq) (SourceStep.hpp:160-197).  A zero term gives J = I's entry
+// without the product: for finite dt that is the same value (0 - dt 0 = +0, 1 - dt 0 = 1), and
+// entries the model's Jacobian fixes at zero become compile-time constants.
+__device__ __forceinline__ void newton_jacobian(const Params& P, T dt, const T* q, const T* a, T* J) {
+    const T X0[3] = {0, 0, 0};
+    auto dSdQ = M::source_jacobian_wrt_variables(q, a, P.p);
+    auto dSdA = M::source_jacobian_wrt_aux_variables(q, a, P.p);
+    // The aux chain term dS/dA dA/dq only where dS/dA has a nonzero entry: for a model whose
+    // dS/dA is identically zero the test folds at compile time and the finite-difference aux
+    // Jacobian is not formed (with a finite dA/dq the term adds a signed zero, which leaves J
+    // unchanged)
+    bool aux_term = false;
+#pragma unroll
+    for (int i = 0; i < NQ * NA; ++i) aux_term |= dSdA[i] != (T)0;
+    T Jx[NA * NQ];
+#pragma unroll
+    for (int i = 0; i < NA * NQ; ++i) Jx[i] = 0;
+    if (aux_term) {
+        auto a0 = M::update_aux_variables(q, a, P.p, 0.0, X0);
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            T qp[NQ];
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) qp[l] = q[l];
+            const T h = (T)1e-7 * (std::fabs(q[j]) + (T)1e-7);
+            qp[j] += h;
+            auto a1 = M::update_aux_variables(qp, a, P.p, 0.0, X0);
+#pragma unroll
+            for (int l = 0; l < NA; ++l) Jx[l * NQ + j] = (a1[l] - a0[l]) / h;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            T term = dSdQ[i * NQ + j];
+            if (aux_term) {
+#pragma unroll
+                for (int l = 0; l < NA; ++l) term += dSdA[i * NA + l] * Jx[l * NQ + j];
+            }
+            J[i * NQ + j] = (i == j ? (T)1.0 : (T)0.0) - (term == (T)0 ? (T)0 : dt * term);
+        }
+}
+
+// Gaussian elimination of A x = b without row exchanges, valid (returns true) where GEPP
+// (SourceStep::SolveLinearSystem, SourceStep.hpp:211-242) would keep every pivot on the diagonal
+// (no |A[i][k]| > |A[k][k]| below it), no pivot is below 1e-14 or non-finite, every multiplier
+// and every x is finite: then x is GEPP's bit for bit.  Zero entries are skipped: a zero below
+// the pivot gives GEPP's multiplier 0 / pivot = copysign(0, pivot) and leaves the row of A as it
+// is (A never holds -0 and the products are finite), a zero in the pivot row or in the back
+// substitution adds a signed zero to a value that is not -0.  Entries the model fixes at zero
+// are compile-time constants, so their registers and arithmetic fold away.
+__device__ __forceinline__ bool ge_nopivot(T* A, T* b, T* x) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const T piv = A[k * NQ + k];
+        const T mv = std::fabs(piv);
+#pragma unroll
+        for (int i = k + 1; i < NQ; ++i) ok &= !(std::fabs(A[i * NQ + k]) > mv);
+        ok &= mv >= (T)1e-14 && mv <= DBL_MAX;
+#pragma unroll
+        for (int i = k + 1; i < NQ; ++i) {
+            const T aik = A[i * NQ + k];
+            const T fac = aik == (T)0 ? std::copysign((T)0, piv) : aik / piv;
+            ok &= std::fabs(fac) <= DBL_MAX;
+#pragma unroll
+            for (int j = k + 1; j < NQ; ++j) {
+                const T akj = A[k * NQ + j];
+                A[i * NQ + j] = akj == (T)0 ? A[i * NQ + j] : A[i * NQ + j] - fac * akj;
+            }
+            b[i] -= fac * b[k];
+        }
+    }
+#pragma unroll
+    for (int i = NQ - 1; i >= 0; --i) {
+        T sum = 0.0;
+#pragma unroll
+        for (int j = i + 1; j < NQ; ++j) sum = A[i * NQ + j] == (T)0 ? sum : sum + A[i * NQ + j] * x[j];
+        x[i] = (b[i] - sum) / A[i * NQ + i];
+        ok &= std::fabs(x[i]) <= DBL_MAX;
+    }
+    return ok;
+}
+
+// The Newton update of one iteration by the full GEPP with row exchanges (SourceStep::
+// SolveLinearSystem, SourceStep.hpp:211-242, the operations of gepp_reg in the same order) for
+// the lanes ge_nopivot declined, which are rare.  Those lanes take turns on one LDS workspace per
+// wavefront (the matrix, right-hand side and solution, 576 bytes at n = 8), so the matrix holds
+// neither registers nor scratch memory.  Each recomputes its residual and J exactly as the caller
+// did.  Workgroups of at most BLOCK threads.
+constexpr int EXCH_WAVES = BLOCK / 64;
+constexpr int EXCH_WS = NQ * NQ + 2 * NQ;
+
+__device__ __forceinline__ bool newton_update_exchange(const Params& P, T dt, const T* q, const T* qo, const T* a,
+                                                       T* dl) {
+    __shared__ T ws_all[EXCH_WAVES][EXCH_WS];
+    T* A = ws_all[threadIdx.x >> 6];
+    T* b = A + NQ * NQ;
+    T* x = b + NQ;
+    const int lane = threadIdx.x & 63;
+    bool res = true;
+    unsigned long long pend = __ballot(1);   // the lanes in here, served one at a time
+    while (pend) {
+        const int L = __ffsll((long long)pend) - 1;
+        pend &= pend - 1;
+        if (lane != L) continue;
+        {
+            // copies behind an empty asm: the compiler must not merge this residual and Jacobian
+            // with the caller's (that would keep the caller's J live in registers)
+            T qq[NQ], qqo[NQ], aa[NA];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                qq[i] = q[i];
+                qqo[i] = qo[i];
+                asm volatile("" : "+v"(qq[i]), "+v"(qqo[i]));
+            }
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+                aa[i] = a[i];
+                asm volatile("" : "+v"(aa[i]));
+            }
+            T r[NQ], J[NQ * NQ];
+            newton_residual(P, dt, qq, qqo, aa, r);
+            newton_jacobian(P, dt, qq, aa, J);
+#pragma unroll
+            for (int i = 0; i < NQ * NQ; ++i) A[i] = J[i];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) b[i] = -r[i];
+        }
+        bool ok = true;
+        #pragma unroll 1
+        for (int k = 0; k < NQ && ok; ++k) {
+            int mr = k;
+            T mv = std::fabs(A[k * NQ + k]);
+            #pragma unroll 1
+            for (int i = k + 1; i < NQ; ++i) {
+                const T v = std::fabs(A[i * NQ + k]);
+                if (v > mv) {
+                    mv = v;
+                    mr = i;
+                }
+            }
+            if (mv < 1e-14) {
+                ok = false;
+                break;
+            }
+            if (mr != k) {
+                #pragma unroll 1
+                for (int j = k; j < NQ; ++j) {
+                    const T t = A[k * NQ + j];
+                    A[k * NQ + j] = A[mr * NQ + j];
+                    A[mr * NQ + j] = t;
+                }
+                const T t = b[k];
+                b[k] = b[mr];
+                b[mr] = t;
+            }
+            #pragma unroll 1
+            for (int i = k + 1; i < NQ; ++i) {
+                const T fac = A[i * NQ + k] / A[k * NQ + k];
+                #pragma unroll 1
+                for (int j = k; j < NQ; ++j) A[i * NQ + j] -= fac * A[k * NQ + j];
+                b[i] -= fac * b[k];
+            }
+        }
+        if (ok) {
+            #pragma unroll 1
+            for (int i = NQ - 1; i >= 0; --i) {
+                T sum = 0.0;
+                #pragma unroll 1
+                for (int j = i + 1; j < NQ; ++j) sum += A[i * NQ + j] * x[j];
+                x[i] = (b[i] - sum) / A[i * NQ + i];
+            }
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) dl[i] = x[i];
+        }
+        res = ok;
+    }
+    return res;
+}
+
 // SourceStep::SolveLocalCell (SourceStep.hpp:125-209): implicit Euler Newton, <= 15
 // iterations, |R|_2 < 1e-8, aux Jacobian by forward differences (SourceStep.hpp:40-50)
 __device__ __forceinline__ void newton_cell(const Params& P, T dt, T* q, const T* aux_init) {
@@ -902,62 +1098,17 @@ __device__ __forceinline__ void newton_cell(const Params& P, T dt, T* q, const T
         auto ua = M::update_aux_variables(q, a, P.p, 0.0, X0);
 #pragma unroll
         for (int i = 0; i < NA; ++i) a[i] = ua[i];
-        auto S = M::source(q, a, P.p, (T)0, dt, X0);
         T r[NQ];
-        T rn = 0.0;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            r[i] = q[i] - qo[i] - dt * S[i];
-            rn += r[i] * r[i];
-        }
+        const T rn = newton_residual(P, dt, q, qo, a, r);
         if (std::sqrt(rn) < 1e-8) break;
-        auto dSdQ = M::source_jacobian_wrt_variables(q, a, P.p);
-        auto dSdA = M::source_jacobian_wrt_aux_variables(q, a, P.p);
-        // The aux chain term dS/dA dA/dq only where dS/dA has a nonzero entry: for a model
-        // whose dS/dA is identically zero the test folds at compile time and the finite-
-        // difference aux Jacobian is not formed (with a finite dA/dq the term adds a signed
-        // zero, which leaves J unchanged)
-        bool aux_term = false;
-#pragma unroll
-        for (int i = 0; i < NQ * NA; ++i) aux_term |= dSdA[i] != (T)0;
-        T Jx[NA * NQ];
-#pragma unroll
-        for (int i = 0; i < NA * NQ; ++i) Jx[i] = 0;
-        if (aux_term) {
-            auto a0 = M::update_aux_variables(q, a, P.p, 0.0, X0);
-#pragma unroll
-            for (int j = 0; j < NQ; ++j) {
-                T qp[NQ];
-#pragma unroll
-                for (int l = 0; l < NQ; ++l) qp[l] = q[l];
-                const T h = (T)1e-7 * (std::fabs(q[j]) + (T)1e-7);
-                qp[j] += h;
-                auto a1 = M::update_aux_variables(qp, a, P.p, 0.0, X0);
-#pragma unroll
-                for (int l = 0; l < NA; ++l) Jx[l * NQ + j] = (a1[l] - a0[l]) / h;
-            }
-        }
         T J[NQ * NQ];
-#pragma unroll
-        for (int i = 0; i < NQ; ++i)
-#pragma unroll
-            for (int j = 0; j < NQ; ++j) {
-                T term = dSdQ[i * NQ + j];
-                if (aux_term) {
-#pragma unroll
-                    for (int l = 0; l < NA; ++l) term += dSdA[i * NA + l] * Jx[l * NQ + j];
-                }
-                J[i * NQ + j] = (i == j ? (T)1.0 : (T)0.0) - dt * term;
-            }
+        newton_jacobian(P, dt, q, a, J);
 #pragma unroll
         for (int i = 0; i < NQ; ++i) r[i] = -r[i];
         T dl[NQ];
-        if (gepp_reg(J, r, dl)) {
+        if (!ge_nopivot(J, r, dl) && !newton_update_exchange(P, dt, q, qo, a, dl)) break;
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) q[i] += dl[i];
-        } else {
-            break;
-        }
+        for (int i = 0; i < NQ; ++i) q[i] += dl[i];
     }
 }
 
