@@ -9,7 +9,10 @@
 // Numerics.H) given on the include path, exactly as the reference is built on its generated
 // headers.
 //
-// Pinning: see tests/test_oracle_*.py (closed forms, invariants, the reference's KATs).
+// Pinning: see tests/test_oracle_*.py (closed forms, invariants, the reference's KATs) and
+// tests/test_reference_vectors.py: the model headers the oracle and the device share (the
+// restated 1-D SME level 0 Model.H / Numerics.H and the VAM ChorinOps.H operators) against
+// vectors evaluated from the reference's generated headers' own expression text.
 // Parity unpinned: PETSc's TSSSP RKS2 / VecAXPBYPCZ rounding order (third-party, restated
 // from its published algorithm), the DMPlex face numbering (the oracle uses the mesh
 // service's ordering).
@@ -36,6 +39,14 @@ template <typename TT>
 struct HasCellSource<TT, std::void_t<decltype(Model<TT>::hydrostatic_cell_source(
                              std::declval<const TT*>(), std::declval<const TT*>(),
                              std::declval<const TT*>(), std::declval<const TT*>()))>>
+    : std::true_type {};
+
+// Model::initial_condition(X, p) is optional in a generated model (VirtualSolver.hpp:33-38)
+template <typename MM, typename = void>
+struct HasInitialConditionO : std::false_type {};
+template <typename MM>
+struct HasInitialConditionO<MM, std::void_t<decltype(MM::initial_condition(std::declval<const double*>(),
+                                                                          std::declval<const double*>()))>>
     : std::true_type {};
 
 namespace {
@@ -982,6 +993,51 @@ int orc_eigen(const double* q, const double* a, const double* p, const double* n
     lam[0] = NUM::local_max_abs_eigenvalue(q, a, p, n)[0];
     return 0;
 }
+
+// The model header's per-point functions at one state (checked against vectors evaluated from
+// the reference's generated Model.H): source, dS/dq (n x n), dS/da (n x n_aux), the clamp, the
+// aux update, the eigenvalues along n, the ghost state of boundary condition bc and the
+// initial condition at X.
+int orc_model_point(const double* Q, const double* A, const double* p, const double* n, const double* X,
+                    double time, double dt, int bc, double* src, double* dsdq, double* dsda, double* upd,
+                    double* aux, double* eig, double* ghost, double* ic) {
+    auto s = M::source(Q, A, p, time, dt, X);
+    auto jq = M::source_jacobian_wrt_variables(Q, A, p);
+    auto ja = M::source_jacobian_wrt_aux_variables(Q, A, p);
+    auto u = M::update_variables(Q, A, p, dt);
+    auto a = M::update_aux_variables(Q, A, p, time, X);
+    auto e = M::eigenvalues(Q, A, p, n);
+    auto g = M::boundary_conditions(bc, Q, A, p, n, X, time, 0.0);
+    for (int i = 0; i < NQ; ++i) {
+        src[i] = s[i];
+        upd[i] = u[i];
+        eig[i] = e[i];
+        ghost[i] = g[i];
+    }
+    for (int i = 0; i < NQ * NQ; ++i) dsdq[i] = jq[i];
+    for (int i = 0; i < NQ * NA; ++i) dsda[i] = ja[i];
+    for (int i = 0; i < NA; ++i) aux[i] = a[i];
+    if constexpr (HasInitialConditionO<M>::value) {
+        auto v = M::initial_condition(X, p);
+        for (int i = 0; i < NQ; ++i) ic[i] = v[i];
+    } else {
+        for (int i = 0; i < NQ; ++i) ic[i] = 0;
+    }
+    return 0;
+}
+
+#ifdef ZOOMY_MODEL_IS_VAM
+// ChorinOps.H's three per-cell operators at one state: the 2x2 pressure blocks (VAM_N_PBLOCK of
+// them, row-major), the pressure residual and the corrected slots (checked against vectors
+// evaluated from the reference's generated ChorinOps.H)
+int orc_vam_point(const double* Q, const double* A, const double* p, double dt, double* blocks, double* res,
+                  double* corr) {
+    vam_pressure_blocks(Q, A, p, dt, blocks);
+    vam_pressure_residual(Q, A, p, dt, res);
+    vam_corrector(Q, A, p, dt, corr);
+    return 0;
+}
+#endif
 
 int orc_update_state(int nc, const double* p, double* X, double* A) {
     update_state(nc, p, X, A);
