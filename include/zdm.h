@@ -49,6 +49,10 @@ typedef struct zdm_options {
                            IMEXStrategy, SolverStrategies.hpp:62-84), 2 "implicit" (BDF2 on
                            transport + source, Newton-Krylov, FullyImplicitStrategy,
                            SolverStrategies.hpp:88-195) */
+    int refresh_derivative_aux; /* solver.refresh_derivative_aux (Settings.hpp:24, default true): for a
+                           model with the field-level update_aux_variables(Q, Qaux, dt, mesh), run it
+                           (RefreshDerivativeAux, TransportStep.hpp:161-202) before every UpdateState;
+                           0 skips it.  Splitting time integration without MOOD only */
 } zdm_options;
 
 /* Newton-Krylov settings of time_integration "implicit" (the SNES / KSP of the TS). */

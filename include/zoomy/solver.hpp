@@ -516,6 +516,7 @@ public:
         o.well_balanced = ss.well_balanced ? 1 : 0;
         o.positivity = positivity;
         o.wet_dry_eps = ss.wet_dry_eps;
+        o.refresh_derivative_aux = ss.refresh_derivative_aux ? 1 : 0;   // ModularSolver.hpp:41
         ZOOMY_TRY(strategy->SetupTS(this, o));
         if (h_) zdm_solver_destroy(h_);
         h_ = nullptr;

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 MODELS_DIR = os.path.join(ROOT, "zoomy-dmplex_amd", "models")
-MODELS = ["sme0_1d", "sme0", "sme1", "sme1m", "sme2", "vam"]
+MODELS = ["sme0_1d", "sme0", "sme0_visc", "sme1", "sme1m", "sme2", "vam"]
 
 
 TANGENT_MODELS = ["sme1m", "vam"]

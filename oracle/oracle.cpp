@@ -27,6 +27,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "zoomy/UserFunctions.H"   // ZoomyMesh, compute_derivative, ...: before the model
 #ifdef ORACLE_TANGENT
 #include "dual.H"   // before the model: its templates name std::sqrt / std::fabs on duals
 #endif
@@ -41,6 +42,42 @@ struct HasCellSource<TT, std::void_t<decltype(Model<TT>::hydrostatic_cell_source
                              std::declval<const TT*>(), std::declval<const TT*>()))>>
     : std::true_type {};
 
+// Model::update_variables (the wet/dry clamp) is optional in a generated model: without it the
+// state is kept (HasUpdateVariables, TransportStep.hpp:23-28, 219)
+template <typename TT, typename = void>
+struct HasUpdateVariablesO : std::false_type {};
+template <typename TT>
+struct HasUpdateVariablesO<TT, std::void_t<decltype(Model<TT>::update_variables(
+                                   std::declval<const TT*>(), std::declval<const TT*>(),
+                                   std::declval<const TT*>(), std::declval<const TT>()))>> : std::true_type {};
+template <typename S>
+SimpleArray<S, Model<S>::n_dof_q> model_update_variables(const S* q, const S* a, const S* p, const S dt) {
+    if constexpr (HasUpdateVariablesO<S>::value) {
+        return Model<S>::update_variables(q, a, p, dt);
+    } else {
+        SimpleArray<S, Model<S>::n_dof_q> r;
+        for (int i = 0; i < Model<S>::n_dof_q; ++i) r[i] = q[i];
+        return r;
+    }
+}
+
+// Model::diffusive_flux(q_face, a_face, gradQ_face, p), optional (TransportStep.hpp:10-18)
+template <typename TT, typename = void>
+struct HasDiffusiveFluxO : std::false_type {};
+template <typename TT>
+struct HasDiffusiveFluxO<TT, std::void_t<decltype(Model<TT>::diffusive_flux(
+                                 std::declval<const TT*>(), std::declval<const TT*>(), std::declval<const TT*>(),
+                                 std::declval<const TT*>()))>> : std::true_type {};
+
+// the field-level, mesh-aware update_aux_variables(Q, Qaux, dt, mesh) (HasFieldAuxUpdate,
+// TransportStep.hpp:30-38)
+template <typename TT, typename = void>
+struct HasFieldAuxUpdateO : std::false_type {};
+template <typename TT>
+struct HasFieldAuxUpdateO<TT, std::void_t<decltype(Model<TT>::update_aux_variables(
+                                  std::declval<TT* const*>(), std::declval<TT* const*>(), std::declval<const TT>(),
+                                  std::declval<const ZoomyMesh&>()))>> : std::true_type {};
+
 // Model::initial_condition(X, p) is optional in a generated model (VirtualSolver.hpp:33-38)
 template <typename MM, typename = void>
 struct HasInitialConditionO : std::false_type {};
@@ -48,6 +85,15 @@ template <typename MM>
 struct HasInitialConditionO<MM, std::void_t<decltype(MM::initial_condition(std::declval<const double*>(),
                                                                           std::declval<const double*>()))>>
     : std::true_type {};
+template <typename MM>
+void initial_condition_point(const double* X, const double* p, double* ic) {
+    if constexpr (HasInitialConditionO<MM>::value) {
+        auto v = MM::initial_condition(X, p);
+        for (int i = 0; i < MM::n_dof_q; ++i) ic[i] = v[i];
+    } else {
+        for (int i = 0; i < MM::n_dof_q; ++i) ic[i] = 0;
+    }
+}
 
 namespace {
 
@@ -99,11 +145,74 @@ void update_state(int nc, const T* p, T* X, T* A) {
     for (int c = 0; c < nc; ++c) {
         T* q = X + (size_t)c * NQ;
         T* a = A + (size_t)c * NA;
-        auto rq = M::update_variables(q, a, p, 0.0);
+        auto rq = model_update_variables<T>(q, a, p, 0.0);
         for (int i = 0; i < NQ; ++i) q[i] = rq[i];
         auto ra = M::update_aux_variables(q, a, p, 0.0, X0);
         for (int i = 0; i < NA; ++i) a[i] = ra[i];
     }
+}
+
+// solver.refresh_derivative_aux (Settings.hpp:24, default true)
+int g_refresh = 1;
+int g_deriv_err = 0;
+
+// compute_derivative on host fields (UserFunctions.H:60-112): the Green-Gauss face loop,
+// 0.5 (s_L + s_R) N scattered into both cells in face order, a zero-Neumann face value s_L on
+// boundary faces, then / volume (0 where the volume is not positive); order 0 copies
+void host_field_derivative(void* ctx, double* result, const double* source, int dx, int dy, int dz) {
+    const zdm_mesh* m = static_cast<const zdm_mesh*>(ctx);
+    const int nc = m->ncells, total = dx + dy + dz;
+    for (int c = 0; c < nc; ++c) result[c] = 0.0;
+    if (total == 0) {
+        for (int c = 0; c < nc; ++c) result[c] = source[c];
+        return;
+    }
+    const int axis = dx == 1 ? 0 : (dy == 1 ? 1 : 2);
+    if (total != 1 || axis >= DIM) {
+        ++g_deriv_err;
+        return;
+    }
+    std::vector<double> g(nc, 0.0);
+    for (int f = 0; f < m->nfaces; ++f) {
+        const double N = m->face_normal[3 * (size_t)f + axis];
+        const int L = m->face_cells[2 * f], R = m->face_cells[2 * f + 1];
+        if (R >= 0) {
+            const double fv = 0.5 * (source[L] + source[R]);
+            const double v = fv * N;
+            g[L] += v;
+            g[R] -= v;
+        } else {
+            g[L] += source[L] * N;
+        }
+    }
+    for (int c = 0; c < nc; ++c)
+        if (m->cell_volume[c] > 0.0) result[c] = g[c] / m->cell_volume[c];
+}
+
+// TransportStep::RefreshDerivativeAux (TransportStep.hpp:161-202): de-interleave X and A into
+// per-field arrays, the model's field-level update_aux_variables(Q, Qaux, 0, mesh), scatter the
+// aux back.  No-op for a model without it or with the setting off.
+template <typename S>
+void refresh_aux(const zdm_mesh* m, const S* X, S* A) {
+    if constexpr (std::is_same<S, double>::value && HasFieldAuxUpdateO<S>::value) {
+        if (!g_refresh) return;
+        const int nc = m->ncells;
+        std::vector<std::vector<S>> Qf(NQ, std::vector<S>(nc)), Af(NA, std::vector<S>(nc));
+        for (int c = 0; c < nc; ++c) {
+            for (int i = 0; i < NQ; ++i) Qf[i][c] = X[(size_t)c * NQ + i];
+            for (int k = 0; k < NA; ++k) Af[k][c] = A[(size_t)c * NA + k];
+        }
+        std::vector<S*> Qp(NQ), Ap(NA);
+        for (int i = 0; i < NQ; ++i) Qp[i] = Qf[i].data();
+        for (int k = 0; k < NA; ++k) Ap[k] = Af[k].data();
+        const ZoomyMesh mesh{const_cast<zdm_mesh*>(m), &host_field_derivative, DIM, 0, (long)nc};
+        Model<S>::update_aux_variables(Qp.data(), Ap.data(), (S)0, mesh);
+        for (int c = 0; c < nc; ++c)
+            for (int k = 0; k < NA; ++k) A[(size_t)c * NA + k] = Af[k][c];
+    }
+    (void)m;
+    (void)X;
+    (void)A;
 }
 
 // TransportStep::UpdateNeighborBounds, TransportStep.hpp:122-154
@@ -412,12 +521,43 @@ void face_state(const T* q, const T* grad, const double* xc, const double* xf, c
 // TransportStep::ComputeFluxes, TransportStep.hpp:383-565 (owned = all cells on one rank)
 // Extension (well_balanced): after a face's flux, each side adds
 // area * Model::hydrostatic_cell_source(q_face, q_cell, p, outward normal).
-void cell_source(int wb, const T* qf, const T* qc, const T* p, const T* nh, T sign, T area, T* f) {
-    if constexpr (HasCellSource<T>::value) {
+template <typename S>
+void cell_source(int wb, const S* qf, const S* qc, const S* p, const S* nh, double sign, S area, S* f) {
+    if constexpr (HasCellSource<S>::value) {
         if (!wb) return;
-        const T no[3] = {sign < 0 ? -nh[0] : nh[0], sign < 0 ? -nh[1] : nh[1], 0};
-        auto S = M::hydrostatic_cell_source(qf, qc, p, no);
-        for (int i = 0; i < NQ; ++i) f[i] += S[i] * area;
+        const S no[3] = {sign < 0 ? -nh[0] : nh[0], sign < 0 ? -nh[1] : nh[1], 0};
+        auto Sv = Model<S>::hydrostatic_cell_source(qf, qc, p, no);
+        for (int i = 0; i < NQ; ++i) f[i] += Sv[i] * area;
+    }
+}
+
+// the diffusive face term (TransportStep.hpp:468-491): averaged state, aux and gradient of the
+// two sides; fL += d area, fR -= d area
+template <typename S>
+void diffusive_face(const S* qL, const S* qR, const S* aL, const S* aR, const S* gL, const S* gR, const S* p,
+                    S area, S* fL, S* fR) {
+    if constexpr (HasDiffusiveFluxO<S>::value) {
+        S gq[NQ * DIM], qf[NQ], af[NA];
+        for (int k = 0; k < NQ * DIM; ++k) gq[k] = 0.5 * (gL[k] + gR[k]);
+        for (int i = 0; i < NQ; ++i) qf[i] = 0.5 * (qL[i] + qR[i]);
+        for (int i = 0; i < NA; ++i) af[i] = 0.5 * (aL[i] + aR[i]);
+        auto d = Model<S>::diffusive_flux(qf, af, gq, p);
+        for (int i = 0; i < NQ; ++i) {
+            fL[i] += d[i] * area;
+            fR[i] -= d[i] * area;
+        }
+    }
+}
+// ... at a boundary face (TransportStep.hpp:533-550): the interior cell's gradient, the state
+// averaged with the ghost, the interior face aux
+template <typename S>
+void diffusive_boundary(const S* qL, const S* qR, const S* aL, const S* gL, const S* p, S area, S* fL) {
+    if constexpr (HasDiffusiveFluxO<S>::value) {
+        S qf[NQ], af[NA];
+        for (int i = 0; i < NQ; ++i) qf[i] = 0.5 * (qL[i] + qR[i]);
+        for (int i = 0; i < NA; ++i) af[i] = aL[i];
+        auto d = Model<S>::diffusive_flux(qf, af, gL, p);
+        for (int i = 0; i < NQ; ++i) fL[i] += d[i] * area;
     }
 }
 
@@ -465,6 +605,7 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
                 fL[i] -= flux[i] * area;
                 fR[i] += flux[i] * area;
             }
+            if (G) diffusive_face<T>(qLf, qRf, aLf, aRf, G + (size_t)cL * NQ * DIM, G + (size_t)cR * NQ * DIM, p, area, fL, fR);
             cell_source(wb, qLf, X + (size_t)cL * NQ, p, nh, 1.0, area, fL);
             cell_source(wb, qRf, X + (size_t)cR * NQ, p, nh, -1.0, area, fR);
         } else if (face_bc[f] >= 0) {
@@ -483,6 +624,7 @@ void compute_fluxes(const zdm_mesh* m, const int* face_bc, const T* p, T time, c
             auto nc = NUM::numerical_fluctuations(qLf, qR.data, aLf, aRf, p, nh);
             for (int i = 0; i < NQ; ++i) fL[i] -= nc[NQ + i] * area;
             for (int i = 0; i < NQ; ++i) fL[i] -= flux[i] * area;
+            if (G) diffusive_boundary<T>(qLf, qR.data, aLf, G + (size_t)cL * NQ * DIM, p, area, fL);
             cell_source(wb, qLf, X + (size_t)cL * NQ, p, nh, 1.0, area, fL);
         } else if (wb) {
             T qLf[NQ];
@@ -511,6 +653,7 @@ void form_rhs(const zdm_mesh* m, const int* face_bc, const T* p, const Opts& o, 
     std::fill(F, F + nc * NQ, 0.0);
     w.Xl.assign(X, X + nc * NQ);
     w.Al.assign(A_in, A_in + nc * NA);
+    refresh_aux(m, w.Xl.data(), w.Al.data());   // UpdateState: the derivative aux of the unclamped X first
     update_state(m->ncells, p, w.Xl.data(), w.Al.data());
     const bool grad = o.order >= 2;
     if (grad) {
@@ -658,6 +801,11 @@ void source_solve(int nc, const T* p, T dt, T* X, const T* A) {
 
 // MUSCLSolver::EnforcePhysicalConstraints, MUSCLSolver.hpp:234-254
 void enforce(int nc, const T* p, T* X, T* A) { update_state(nc, p, X, A); }
+// ... with the RefreshDerivativeAux of TransportStep::UpdateState (TransportStep.hpp:202)
+void enforce(const zdm_mesh* m, const T* p, T* X, T* A) {
+    refresh_aux(m, X, A);
+    update_state(m->ncells, p, X, A);
+}
 
 // One TSStep: PETSc TSSSP "rks2" (s stages) for order >= 2, TSEULER otherwise
 // (SolverStrategies.hpp:26-38).  SSP-RK(s,2):
@@ -1004,7 +1152,7 @@ int orc_model_point(const double* Q, const double* A, const double* p, const dou
     auto s = M::source(Q, A, p, time, dt, X);
     auto jq = M::source_jacobian_wrt_variables(Q, A, p);
     auto ja = M::source_jacobian_wrt_aux_variables(Q, A, p);
-    auto u = M::update_variables(Q, A, p, dt);
+    auto u = model_update_variables<T>(Q, A, p, dt);
     auto a = M::update_aux_variables(Q, A, p, time, X);
     auto e = M::eigenvalues(Q, A, p, n);
     auto g = M::boundary_conditions(bc, Q, A, p, n, X, time, 0.0);
@@ -1017,12 +1165,7 @@ int orc_model_point(const double* Q, const double* A, const double* p, const dou
     for (int i = 0; i < NQ * NQ; ++i) dsdq[i] = jq[i];
     for (int i = 0; i < NQ * NA; ++i) dsda[i] = ja[i];
     for (int i = 0; i < NA; ++i) aux[i] = a[i];
-    if constexpr (HasInitialConditionO<M>::value) {
-        auto v = M::initial_condition(X, p);
-        for (int i = 0; i < NQ; ++i) ic[i] = v[i];
-    } else {
-        for (int i = 0; i < NQ; ++i) ic[i] = 0;
-    }
+    initial_condition_point<M>(X, p, ic);
     return 0;
 }
 
@@ -1041,6 +1184,18 @@ int orc_vam_point(const double* Q, const double* A, const double* p, double dt, 
 
 int orc_update_state(int nc, const double* p, double* X, double* A) {
     update_state(nc, p, X, A);
+    return 0;
+}
+
+// EnforcePhysicalConstraints on the mesh (RefreshDerivativeAux + clamp + aux)
+int orc_enforce(const zdm_mesh* m, const double* p, double* X, double* A) {
+    g_deriv_err = 0;
+    enforce(m, p, X, A);
+    return g_deriv_err ? -1 : 0;
+}
+
+int orc_set_refresh(int on) {
+    g_refresh = on != 0;
     return 0;
 }
 
@@ -1142,7 +1297,7 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
     double t = *time;
     double dt = *dt_io;
     if (first) {
-        enforce(m->ncells, p, X, A);
+        enforce(m, p, X, A);
         dt = std::max(compute_time_step(m, p, X, A, cfl, nullptr), min_dt);
         g_bdf_have = false;
     }
@@ -1160,7 +1315,7 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
             bdf_step(m, face_bc, p, o, t0, dt0, X, A, w);
         else
             ts_step(m, face_bc, p, o, t0, dt0, X, A, w);
-        enforce(m->ncells, p, X, A);
+        enforce(m, p, X, A);
         if (g_positivity == 1) {
             mask.assign(m->ncells, 0);
             int nt = 0;
@@ -1178,14 +1333,14 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
                     if (mask[c])
                         for (int i = 0; i < NQ; ++i)
                             X[(size_t)c * NQ + i] = Xs[(size_t)c * NQ + i] + dt0 * Fm[(size_t)c * NQ + i];
-                enforce(m->ncells, p, X, A);
+                enforce(m, p, X, A);
                 g_troubled += nt;
                 g_last_mask = mask;
             }
         }
         if (g_time_integration == 0) {
             source_solve(m->ncells, p, dt0, X, A);
-            enforce(m->ncells, p, X, A);
+            enforce(m, p, X, A);
         }
         ++steps;
         t = t0 + dt0;

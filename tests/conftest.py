@@ -57,6 +57,8 @@ class Oracle:
         L.orc_flux.argtypes = [_dp] * 8
         L.orc_eigen.argtypes = [_dp] * 5
         L.orc_update_state.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_enforce.argtypes = [M, _dp, _dp, _dp]
+        L.orc_set_refresh.argtypes = [ctypes.c_int]
         L.orc_compute_dt.argtypes = [M, _dp, _dp, _dp, ctypes.c_double, _ip]
         L.orc_compute_dt.restype = ctypes.c_double
         L.orc_source.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
@@ -110,14 +112,16 @@ class Oracle:
         return out[0]
 
     def form_rhs(self, mesh, face_bc, p, X, A, order=2, use_limiters=1, gradient=0, t=0.0,
-                 well_balanced=0, positivity="none", wet_dry_eps=1e-2):
+                 well_balanced=0, positivity="none", wet_dry_eps=1e-2, refresh_derivative_aux=True):
         F = np.zeros_like(X)
         self.L.orc_set_positivity(self.POSITIVITY[positivity], float(wet_dry_eps))
+        self.L.orc_set_refresh(int(refresh_derivative_aux))
         try:
             self.L.orc_form_rhs(mesh.handle, ip(face_bc), dp(p), order, use_limiters, gradient,
                                 int(well_balanced), t, dp(X), dp(A), dp(F))
         finally:
             self.L.orc_set_positivity(0, 1e-2)
+            self.L.orc_set_refresh(1)
         return F
 
     def wb_faces(self, mesh, face_bc, p, X, A, skip_theta, wet_dry_eps=1e-2):
@@ -141,6 +145,16 @@ class Oracle:
     def update_state(self, p, X, A):
         self.L.orc_update_state(X.shape[0], dp(p), dp(X), dp(A))
 
+    def enforce(self, mesh, p, X, A, refresh_derivative_aux=True):
+        """EnforcePhysicalConstraints: RefreshDerivativeAux (models with the field-level
+        update_aux_variables) + clamp + aux."""
+        self.L.orc_set_refresh(int(refresh_derivative_aux))
+        try:
+            if self.L.orc_enforce(mesh.handle, dp(p), dp(X), dp(A)) != 0:
+                raise RuntimeError("oracle: compute_derivative order not supported")
+        finally:
+            self.L.orc_set_refresh(1)
+
     def source(self, p, dt, X, A):
         self.L.orc_source(X.shape[0], dp(p), float(dt), dp(X), dp(A))
 
@@ -152,12 +166,14 @@ class Oracle:
 
     def run(self, mesh, face_bc, p, X, A, order=2, ssp_stages=2, use_limiters=1, gradient=0,
             cfl=0.45, t_end=1e9, min_dt=1e-12, max_steps=10, time=0.0, dt=0.0, first=True,
-            well_balanced=0, positivity="none", wet_dry_eps=1e-2, time_integration="splitting"):
+            well_balanced=0, positivity="none", wet_dry_eps=1e-2, time_integration="splitting",
+            refresh_derivative_aux=True):
         """MUSCLSolver loop; returns (steps, time, next dt).  self.troubled = MOOD overrides."""
         t = np.array([time])
         d = np.array([dt])
         self.L.orc_set_positivity(self.POSITIVITY[positivity], float(wet_dry_eps))
         self.L.orc_set_time_integration({"splitting": 0, "imex": 1, "implicit": 2}[time_integration])
+        self.L.orc_set_refresh(int(refresh_derivative_aux))
         try:
             n = self.L.orc_run(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters,
                                gradient, int(well_balanced), cfl, t_end, min_dt, max_steps, dp(X),
@@ -169,6 +185,7 @@ class Oracle:
         finally:
             self.L.orc_set_positivity(0, 1e-2)
             self.L.orc_set_time_integration(0)
+            self.L.orc_set_refresh(1)
         return n, t[0], d[0]
 
 

@@ -3,6 +3,7 @@
 // duals through the same model functions.  Built per model directory:
 //   g++ -std=c++17 -O1 -ffp-contract=off -I <model_dir> -I models flux_vjp_check.cpp
 // Prints one line per case and "max_rel <x>" at the end; exits 1 on a mismatch above 1e-11.
+#include "zoomy/UserFunctions.H"
 #include "dual.H"
 #include "Numerics.H"
 #include <cstdio>

@@ -9,7 +9,7 @@ import pytest
 
 from conftest import ROOT, zdm
 
-MODELS = ["sme0_1d", "sme0", "sme1", "sme1m", "sme2", "vam"]
+MODELS = ["sme0_1d", "sme0", "sme0_visc", "sme1", "sme1m", "sme2", "vam"]
 
 
 def declared(header):

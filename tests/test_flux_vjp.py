@@ -21,7 +21,8 @@ def test_flux_vjp_matches_dual_jacobian(model, tmp_path):
         pytest.skip("g++ not available")
     exe = str(tmp_path / f"fvc_{model}")
     subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-I", os.path.join(MODELS, model),
-                    "-I", MODELS, SRC, "-o", exe], check=True, capture_output=True, text=True)
+                    "-I", MODELS, "-I", os.path.join(ROOT, "include"), SRC, "-o", exe], check=True,
+                   capture_output=True, text=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     last = r.stdout.strip().splitlines()[-1]

@@ -99,7 +99,7 @@ def test_source_and_cfl_match_oracle(model):
     s.enforce_constraints()
     Xe, Ae = s.download()
     Xc, Ac = X.copy(), A.copy()
-    orc.update_state(s.params, Xc, Ac)
+    orc.enforce(mesh, s.params, Xc, Ac)   # incl. RefreshDerivativeAux for the 1-D model
     assert np.array_equal(Xe, Xc) and np.array_equal(Ae, Ac)
     dt_g = s.cfl_dt()
     dt_c = orc.compute_dt(mesh, s.params, Xc, Ac, 0.45)

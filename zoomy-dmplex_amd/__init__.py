@@ -277,7 +277,8 @@ class _Options(ctypes.Structure):
                 ("source", ctypes.c_int), ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
                 ("min_dt", ctypes.c_double), ("block_size", ctypes.c_int),
                 ("well_balanced", ctypes.c_int), ("positivity", ctypes.c_int),
-                ("wet_dry_eps", ctypes.c_double), ("time_integration", ctypes.c_int)]
+                ("wet_dry_eps", ctypes.c_double), ("time_integration", ctypes.c_int),
+                ("refresh_derivative_aux", ctypes.c_int)]
 
 POSITIVITY = {"none": 0, "mood": 1, "zhang_shu": 2}
 TIME_INTEGRATION = {"splitting": 0, "imex": 1, "implicit": 2}
@@ -421,7 +422,7 @@ class Solver:
     def __init__(self, model, mesh, settings=None, params=None, face_bc=None, source=True,
                  order=None, ssp_stages=None, cfl=None, t_end=None, gradient=None,
                  use_limiters=None, well_balanced=None, cell_offsets=None, positivity=None,
-                 wet_dry_eps=None, time_integration=None):
+                 wet_dry_eps=None, time_integration=None, refresh_derivative_aux=None):
         self.model = model
         self.info = ModelInfo(model)
         self.lib = model_lib(model)
@@ -470,6 +471,8 @@ class Solver:
             raise ZdmError("positivity must be one of %s" % sorted(POSITIVITY))
         o.positivity = POSITIVITY[pos]
         o.wet_dry_eps = float(wet_dry_eps if wet_dry_eps is not None else sv.get("wet_dry_eps", 1e-2))
+        o.refresh_derivative_aux = int(bool(refresh_derivative_aux if refresh_derivative_aux is not None
+                                            else sv.get("refresh_derivative_aux", True)))
         self.options = o
         h = ctypes.c_void_p()
         if cell_offsets is None:

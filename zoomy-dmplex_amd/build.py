@@ -15,7 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
-MODELS = ["sme0_1d", "sme0", "sme1", "sme1m", "sme2", "vam"]
+MODELS = ["sme0_1d", "sme0", "sme0_visc", "sme1", "sme1m", "sme2", "vam"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

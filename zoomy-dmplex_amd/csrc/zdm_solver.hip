@@ -32,6 +32,7 @@
 #include "zdm.h"
 
 #pragma clang force_cuda_host_device begin
+#include "zoomy/UserFunctions.H"   // ZoomyMesh, compute_derivative, eigen kernels: before the model
 #include "dual.H"   // before the model: its templates name std::sqrt / std::fabs on duals
 #include "Numerics.H"
 #ifdef ZOOMY_MODEL_IS_VAM
@@ -50,6 +51,46 @@ template <typename TT>
 struct HasInitialCondition<TT, std::void_t<decltype(Model<TT>::initial_condition(
                                    std::declval<const TT*>(), std::declval<const TT*>()))>>
     : std::true_type {};
+
+// Model::update_variables (the wet/dry state clamp) is optional: a model without it keeps its
+// state (HasUpdateVariables, TransportStep.hpp:23-28, 219)
+template <typename TT, typename = void>
+struct HasUpdateVariables : std::false_type {};
+template <typename TT>
+struct HasUpdateVariables<TT, std::void_t<decltype(Model<TT>::update_variables(
+                                  std::declval<const TT*>(), std::declval<const TT*>(),
+                                  std::declval<const TT*>(), std::declval<const TT>()))>> : std::true_type {};
+
+template <typename S>
+__host__ __device__ __forceinline__ SimpleArray<S, Model<S>::n_dof_q> model_update_variables(const S* q, const S* a,
+                                                                                              const S* p, const S dt) {
+    if constexpr (HasUpdateVariables<S>::value) {
+        return Model<S>::update_variables(q, a, p, dt);
+    } else {
+        SimpleArray<S, Model<S>::n_dof_q> r;
+        for (int i = 0; i < Model<S>::n_dof_q; ++i) r[i] = q[i];
+        return r;
+    }
+}
+
+// Model::diffusive_flux(q_face, a_face, gradQ_face, p), optional (HasDiffusiveFlux,
+// TransportStep.hpp:10-18): the face term added after the flux
+template <typename TT, typename = void>
+struct HasDiffusiveFlux : std::false_type {};
+template <typename TT>
+struct HasDiffusiveFlux<TT, std::void_t<decltype(Model<TT>::diffusive_flux(
+                                std::declval<const TT*>(), std::declval<const TT*>(), std::declval<const TT*>(),
+                                std::declval<const TT*>()))>> : std::true_type {};
+
+// The field-level, mesh-aware update_aux_variables(Q, Qaux, dt, mesh) of a generated model
+// (Model.H:250-262), which fills the spatial-derivative aux through compute_derivative
+// (HasFieldAuxUpdate, TransportStep.hpp:30-38)
+template <typename TT, typename = void>
+struct HasFieldAuxUpdate : std::false_type {};
+template <typename TT>
+struct HasFieldAuxUpdate<TT, std::void_t<decltype(Model<TT>::update_aux_variables(
+                                 std::declval<TT* const*>(), std::declval<TT* const*>(), std::declval<const TT>(),
+                                 std::declval<const ZoomyMesh&>()))>> : std::true_type {};
 
 // Optional numerics hook (not emitted by the reference generator): numerical_flux and
 // numerical_fluctuations from one evaluation, with results identical to the two calls
@@ -124,6 +165,19 @@ thread_local std::string g_err;
 int fail(const std::string& m) {
     g_err = m;
     return -1;
+}
+
+// Model::initial_condition(X, p) at one point (optional in a generated model,
+// VirtualSolver.hpp:33-38), or an error for a model without it
+template <typename S>
+int model_initial_condition(const S* X, const S* p, S* Q) {
+    if constexpr (HasInitialCondition<S>::value) {
+        auto q = Model<S>::initial_condition(X, p);
+        for (int i = 0; i < NQ; ++i) Q[i] = q[i];
+        return 0;
+    } else {
+        return fail("No io.initial_condition_file set and the model emits no initial_condition(x,p)");
+    }
 }
 #define HIPCHK(x)                                                                     \
     do {                                                                              \
@@ -259,7 +313,7 @@ __device__ __forceinline__ void store_soa(T* __restrict__ a, int ld, int c, cons
 // TransportStep::UpdateState per cell (TransportStep.hpp:210-227)
 __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
     const T X0[3] = {0, 0, 0};
-    auto rq = M::update_variables(q, a, P.p, 0.0);
+    auto rq = model_update_variables<T>(q, a, P.p, 0.0);
 #pragma unroll
     for (int i = 0; i < NQ; ++i) q[i] = rq[i];
     auto ra = M::update_aux_variables(q, a, P.p, 0.0, X0);
@@ -283,7 +337,7 @@ __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
 template <int MAXF, int GRAD, typename Put, int AMODE = 0>
 __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, const T* __restrict__ X,
                                            const T* __restrict__ Ag, int c, int use_limiters, T* ac, Put put,
-                                           T wet_eps = 0, int zs_theta = 0, T* abuf = nullptr) {
+                                           T wet_eps = 0, int zs_theta = 0, T* abuf = nullptr, T* gout = nullptr) {
     constexpr bool GG = GRAD == 1 || GRAD == 3;
     const int ld = m.ld;
     T q[NQ], qc[NQ];
@@ -343,7 +397,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
             const int nb = nbk[k];
             const T* N = Nk[k];
             if (nb >= 0) {
-                auto rn = M::update_variables(qn[k], an[k], P.p, 0.0);
+                auto rn = model_update_variables<T>(qn[k], an[k], P.p, 0.0);
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
                     const T w = (GRAD == 3 && i == 1) ? rn[0] + rn[1] : rn[i];
@@ -385,6 +439,13 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
             if (vol > 1e-15) {
 #pragma unroll
                 for (int i = 0; i < NQ * DIM; ++i) g[i] /= vol;
+            }
+        }
+        if constexpr (HasDiffusiveFlux<T>::value && GRAD != 3) {
+            // the cell gradient the diffusive face term reads (TransportStep.hpp:470-475)
+            if (gout) {
+#pragma unroll
+                for (int i = 0; i < NQ * DIM; ++i) gout[(size_t)i * ld + c] = g[i];
             }
         }
         if constexpr (GRAD == 3) {
@@ -538,7 +599,7 @@ template <int MAXF, int GRAD, bool MC>
 __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 const T* __restrict__ X, const T* __restrict__ Ag,
                                                 T* __restrict__ As, T* __restrict__ QF, int use_limiters,
-                                                T wet_eps, int zs_theta) {
+                                                T wet_eps, int zs_theta, T* __restrict__ G) {
     const auto L = locate<MC>(m);
     ctrl += L.m;
     if (!ctrl->active) return;
@@ -547,7 +608,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
         T ac[NA];
         recon_cell<MAXF, GRAD>(
             m, P, X, Ag, c, use_limiters, ac, [&](int k, int i, T v) { QF[qf_idx(ld, k, c, i)] = v; },
-            wet_eps, zs_theta);
+            wet_eps, zs_theta, nullptr, G);
         store_soa<NA>(As, ld, c, ac);
     };
     if constexpr (MC) {
@@ -625,13 +686,67 @@ constexpr double C2 = 2.0 * G;                   // c = (0, 2 g, 1)
 enum { MODE_RHS = 0, MODE_EULER = 1, MODE_STAGE = 2, MODE_FINAL = 3, MODE_IMEX3 = 4 };
 
 // One face slot of the face loop of ComputeFluxes (TransportStep.hpp:383-565) for a slot with
+// fc += area * Model::hydrostatic_cell_source(q_face, q_cell, p, n_out) (the well_balanced
+// extension); nothing for a model without the hook
+template <typename S>
+__device__ __forceinline__ void add_cell_source(const S* qf, const S* qcell, const S* p, const S* no, S area, S* fc) {
+    if constexpr (HasCellSource<S>::value) {
+        auto Sv = Model<S>::hydrostatic_cell_source(qf, qcell, p, no);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) fc[i] += Sv[i] * area;
+    }
+}
+
+// The diffusive face term of a model with Model::diffusive_flux (TransportStep.hpp:468-491):
+// state, aux and gradient averaged over the two sides, the term times the area added to the
+// face's support[0] cell and subtracted from its support[1] cell (right = this cell is the
+// latter).  Nothing for a model without the hook.
+template <typename S>
+__device__ __forceinline__ void add_diffusive_face(const S* qL, const S* qR, const S* aL, const S* aR, const S* gL,
+                                                   const S* gR, const S* p, S area, bool right, S* fc) {
+    if constexpr (HasDiffusiveFlux<S>::value) {
+        S gq[NQ * DIM], qf[NQ], af[NA];
+#pragma unroll
+        for (int k = 0; k < NQ * DIM; ++k) gq[k] = 0.5 * (gL[k] + gR[k]);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) qf[i] = 0.5 * (qL[i] + qR[i]);
+#pragma unroll
+        for (int i = 0; i < NA; ++i) af[i] = 0.5 * (aL[i] + aR[i]);
+        auto d = Model<S>::diffusive_flux(qf, af, gq, p);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            if (right)
+                fc[i] -= d[i] * area;
+            else
+                fc[i] += d[i] * area;
+        }
+    }
+}
+// ... at a boundary face (TransportStep.hpp:533-550): the cell's own gradient, the state
+// averaged with the ghost, the cell side's face aux
+template <typename S>
+__device__ __forceinline__ void add_diffusive_boundary(const S* qL, const S* qR, const S* aL, const S* gL,
+                                                       const S* p, S area, S* fc) {
+    if constexpr (HasDiffusiveFlux<S>::value) {
+        S qf[NQ], af[NA];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) qf[i] = 0.5 * (qL[i] + qR[i]);
+#pragma unroll
+        for (int i = 0; i < NA; ++i) af[i] = aL[i];
+        auto d = Model<S>::diffusive_flux(qf, af, gL, p);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) fc[i] += d[i] * area;
+    }
+}
+
 // f >= 0 and area > 1e-15: the face aux, flux and fluctuations of the (own qo, neighbour qn with
 // clamped aux an) face states and the -D A -/+ F A accumulation into fc, or the boundary ghost
 // (nb < 0), then the optional well-balanced cell term.  meta = the slot's cmeta entry (read only
 // when nb >= 0); ac = the cell's clamped aux, qcell = its clamped state (well-balanced term only).
 __device__ __forceinline__ void flux_face(const DevMesh& m, const Params& P, int f, int nb, int meta, T area,
                                           const T* nh, const T* xf, const T* qo, const T* qn, const T* an,
-                                          const T* ac, T time, int wb, const T* qcell, T* fc) {
+                                          const T* ac, T time, int wb, const T* qcell, T* fc,
+                                          const T* go = nullptr, const T* gn = nullptr) {
     if (nb >= 0) {
         const bool right = (meta >> 2) & 1;
         const T* qL = right ? qn : qo;
@@ -655,6 +770,7 @@ __device__ __forceinline__ void flux_face(const DevMesh& m, const Params& P, int
                 fc[i] -= F[i] * area;
             }
         }
+        if (go) add_diffusive_face<T>(qL, qR, aL.data, aR.data, right ? gn : go, right ? go : gn, P.p, area, right, fc);
     } else {
         const int bc = m.fbc[f];
         if (bc >= 0) {
@@ -668,16 +784,13 @@ __device__ __forceinline__ void flux_face(const DevMesh& m, const Params& P, int
                 fc[i] -= D[NQ + i] * area;
                 fc[i] -= F[i] * area;
             }
+            if (go) add_diffusive_boundary<T>(qo, qR.data, aL.data, go, P.p, area, fc);
         }
     }
-    if constexpr (HasCellSource<T>::value) {
-        if (wb) {
-            const bool right = nb >= 0 && ((meta >> 2) & 1);
-            const T no[3] = {right ? -nh[0] : nh[0], right ? -nh[1] : nh[1], 0};
-            auto S = M::hydrostatic_cell_source(qo, qcell, P.p, no);
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) fc[i] += S[i] * area;
-        }
+    if (wb) {
+        const bool right = nb >= 0 && ((meta >> 2) & 1);
+        const T no[3] = {right ? -nh[0] : nh[0], right ? -nh[1] : nh[1], 0};
+        add_cell_source<T>(qo, qcell, P.p, no, area, fc);
     }
 }
 
@@ -725,16 +838,22 @@ __device__ __forceinline__ void flux_cell(const DevMesh& m, const Params& P, int
 // round trip per slot in place of the dependent chain cface -> farea -> cnbr -> cmeta ->
 // neighbour state.  (Prefetching the next slot during this one's flux needs 156 VGPRs, one wave
 // per SIMD less, and measures slower.)
+constexpr int NGD = HasDiffusiveFlux<T>::value ? NQ * DIM : 1;   // gradient of a diffusive model
 struct FaceIn {
     T area, nh[3], xf[3];
     T qo[NQ], qn[NQ], an[NA];
+    T gn[NGD];   // the neighbour's cell gradient (diffusive models only)
 };
 
 template <int MAXF>
 __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params& P, int c, T time, const T* ac,
                                                  int wb, const T* qcell, const T* __restrict__ QF,
-                                                 const T* __restrict__ As, T* fc) {
+                                                 const T* __restrict__ As, T* fc, const T* __restrict__ G = nullptr) {
     const int ld = m.ld;
+    T go[NGD];   // this cell's gradient (diffusive models only)
+    if constexpr (HasDiffusiveFlux<T>::value) {
+        if (G) load_soa<NGD>(G, ld, c, go);
+    }
     int fk[MAXF], nk[MAXF], mk[MAXF];
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) {
@@ -761,6 +880,9 @@ __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params&
 #pragma unroll
         for (int i = 0; i < NQ; ++i) in.qn[i] = QF[qf_idx(ld, ks, nb, i)];
         load_soa<NA>(As, ld, nb, in.an);
+        if constexpr (HasDiffusiveFlux<T>::value) {
+            if (G) load_soa<NGD>(G, ld, nb, in.gn);
+        }
     };
 #pragma unroll
     for (int i = 0; i < NQ; ++i) fc[i] = 0;
@@ -770,7 +892,7 @@ __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params&
         load(k, in);
         if (fk[k] >= 0 && in.area > 1e-15)
             flux_face(m, P, fk[k], nk[k], mk[k], in.area, in.nh, in.xf, in.qo, in.qn, in.an, ac, time, wb, qcell,
-                      fc);
+                      fc, HasDiffusiveFlux<T>::value && G ? go : nullptr, in.gn);
     }
     flux_divide_volume(m, c, fc);
 }
@@ -799,11 +921,12 @@ __device__ __forceinline__ void stage_update(const Ctrl* ctrl, int c, int ld, co
     }
 }
 
-template <int MAXF, int MODE, bool MC>
+template <int MAXF, int MODE, bool MC, typename TT = T>
 __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 int stage, const T* __restrict__ QF,
                                                 const T* __restrict__ As, const T* In, T* Out,
-                                                const T* __restrict__ Ag, int wb, const T* X0) {
+                                                const T* __restrict__ Ag, int wb, const T* X0,
+                                                const T* __restrict__ G) {
     const auto L = locate<MC>(m);
     ctrl += L.m;
     if (!ctrl->active) return;
@@ -813,17 +936,17 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
         T fc[NQ], ac[NA];
         load_soa<NA>(As, ld, c, ac);
         T qcell[NQ];   // clamped cell state (only read by the well-balanced term)
-        if constexpr (HasCellSource<T>::value) {
+        if constexpr (HasCellSource<TT>::value) {
             if (wb) {
                 T a0[NA];
                 load_soa<NQ>(In, ld, c, qcell);
                 load_soa<NA>(Ag, ld, c, a0);
-                auto rq = M::update_variables(qcell, a0, P.p, 0.0);
+                auto rq = model_update_variables<T>(qcell, a0, P.p, 0.0);
     #pragma unroll
                 for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
             }
         }
-        flux_cell_gather<MAXF>(m, P, c, time, ac, wb, qcell, QF, As, fc);
+        flux_cell_gather<MAXF>(m, P, c, time, ac, wb, qcell, QF, As, fc, G);
         stage_update<MODE>(ctrl, c, ld, fc, In, Out, X0);
     };
     if constexpr (MC) {
@@ -1145,7 +1268,8 @@ __device__ __forceinline__ T block_min(T v) {
 
 // POST: 0 = CFL only, 1 = constraints + CFL, 2 = constraints + source + constraints + CFL,
 //       3 = source only (no CFL), 4 = constraints only (no CFL),
-//       5 = source with the step's dt + constraints (no CFL; the Chorin predictor)
+//       5 = source with the step's dt + constraints (no CFL; the Chorin predictor),
+//       6 = source with the step's dt (no constraints, no CFL)
 template <int POST, bool MC>
 __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl* __restrict__ ctrl, T* X,
                                                 T* A, T* __restrict__ partials, T dt_override) {
@@ -1159,7 +1283,7 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
         load_soa<NQ>(X, ld, c, q);
         load_soa<NA>(A, ld, c, a);
         if constexpr (POST == 1 || POST == 2 || POST == 4) update_state_cell(P, q, a);
-        if constexpr (POST == 2 || POST == 3 || POST == 5) {
+        if constexpr (POST == 2 || POST == 3 || POST == 5 || POST == 6) {
             const T dt = POST == 3 ? dt_override : ctrl->dt;
             newton_cell(P, dt, q, a);
         }
@@ -1445,6 +1569,45 @@ __global__ void k_fill(T* __restrict__ a, size_t n, T v) {
     if (i < n) a[i] = v;
 }
 
+// Green-Gauss derivative along `axis` at cell c of a field given by get(cell), with a
+// zero-Neumann boundary (face value = cell value) and ascending face order, then / vol
+// (compute_derivative, UserFunctions.H:60-112)
+template <int MAXF, typename GET>
+__device__ __forceinline__ T gg_deriv(const DevMesh& m, int c, int axis, GET&& get) {
+    const int ld = m.ld;
+    const T fc = get(c);
+    T g = 0;
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) {
+        const int f = m.cface[(size_t)k * ld + c];
+        if (f < 0) continue;
+        const int nb = m.cnbr[(size_t)k * ld + c];
+        const T N = axis == 0 ? m.fNx[f] : m.fNy[f];
+        if (nb >= 0) {
+            const T fv = (T)0.5 * (fc + get(nb));
+            const T v = fv * N;
+            if ((m.cmeta[(size_t)k * ld + c] >> 2) & 1)
+                g -= v;
+            else
+                g += v;
+        } else {
+            g += fc * N;
+        }
+    }
+    const T vol = m.vol[c];
+    return vol > (T)0 ? g / vol : (T)0;
+}
+
+// out = the derivative along `axis` of the per-cell field src (the device side of
+// compute_derivative for the field-level update_aux_variables)
+template <int MAXF>
+__global__ __launch_bounds__(BLOCK) void k_deriv1(DevMesh m, const T* __restrict__ src, int axis,
+                                                  T* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.ncells) return;
+    out[c] = gg_deriv<MAXF>(m, c, axis, [&](int cc) { return src[cc]; });
+}
+
 #include "krylov.inc"
 #include "imex.inc"
 #include "implicit.inc"
@@ -1467,6 +1630,12 @@ void implicit_free(ImplicitHost* im);
 struct zdm_solver {
     zdm_options opt;
     Params P;
+    bool refresh = false;             // RefreshDerivativeAux before every UpdateState (the model has
+                                      // the field-level update_aux_variables and the setting is on)
+    T* Aref = nullptr;                // the aux a transport stage reads after the refresh
+    T* G = nullptr;                   // cell gradients [NQ DIM][ld] of a model with diffusive_flux
+    int deriv_err = 0;                // compute_derivative failures of the last refresh
+    std::string deriv_msg;
     int ncells, nfaces, ld, maxf, nblocks;
     int bs;                           // workgroup size of the transport kernels (64, 128, 256)
     int ncb;                          // workgroups of 256 over all cells (per-cell kernels via cell_index)
@@ -1503,6 +1672,68 @@ struct zdm_solver {
 namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// the gradient buffer k_prep writes and k_flux reads for a model with diffusive_flux at order 2
+// (the reference computes the gradient only with a reconstruction, TransportStep.hpp:336-345)
+T* grad_buffer(const zdm_solver* s) {
+    return HasDiffusiveFlux<T>::value && s->opt.order >= 2 ? s->G : nullptr;
+}
+
+// ZoomyMesh::derivative of the solver: compute_derivative(result, source, dx, dy, dz) of the
+// model's field-level update_aux_variables on device fields, ordered on the solver stream.
+// Order 0 copies; a first derivative is the Green-Gauss kernel; other orders (which the
+// reference does not support either, UserFunctions.H:62-65) are recorded as an error.
+void field_derivative(void* ctx, double* result, const double* source, int dx, int dy, int dz) {
+    zdm_solver* s = static_cast<zdm_solver*>(ctx);
+    const int total = dx + dy + dz;
+    if (total == 0) {
+        if (hipMemcpyAsync(result, source, sizeof(T) * s->ncells, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
+            ++s->deriv_err;
+            s->deriv_msg = "compute_derivative: device copy failed";
+        }
+        return;
+    }
+    const int axis = dx == 1 ? 0 : (dy == 1 ? 1 : 2);
+    if (dx < 0 || dy < 0 || dz < 0 || total != 1 || axis >= DIM) {
+        ++s->deriv_err;
+        s->deriv_msg = "compute_derivative: only first derivatives along the mesh's axes are supported (D^(" +
+                       std::to_string(dx) + "," + std::to_string(dy) + "," + std::to_string(dz) + ") requested)";
+        return;
+    }
+    const unsigned g = (unsigned)((s->ncells + 255) / 256);
+    if (s->maxf <= 2)
+        k_deriv1<2><<<g, 256, 0, s->stream>>>(s->dm, source, axis, result);
+    else if (s->maxf == 3)
+        k_deriv1<3><<<g, 256, 0, s->stream>>>(s->dm, source, axis, result);
+    else
+        k_deriv1<4><<<g, 256, 0, s->stream>>>(s->dm, source, axis, result);
+    if (hipGetLastError() != hipSuccess) {
+        ++s->deriv_err;
+        s->deriv_msg = "compute_derivative: kernel launch failed";
+    }
+}
+
+// TransportStep::RefreshDerivativeAux (TransportStep.hpp:161-202): the model's field-level
+// update_aux_variables(Q, Qaux, 0, mesh) over the per-field planes of X and A.  The solver's
+// structure-of-arrays planes are the per-field arrays the overload expects, so nothing is
+// de-interleaved; every compute_derivative is a kernel on the solver stream.
+template <typename S>
+int refresh_aux(zdm_solver* s, const S* X, S* A) {
+    if constexpr (HasFieldAuxUpdate<S>::value) {
+        S* Qp[NQ];
+        S* Ap[NA];
+        for (int i = 0; i < NQ; ++i) Qp[i] = const_cast<S*>(X) + (size_t)i * s->ld;
+        for (int k = 0; k < NA; ++k) Ap[k] = A + (size_t)k * s->ld;
+        const ZoomyMesh mesh{s, &field_derivative, DIM, 0, (long)s->ncells};
+        s->deriv_err = 0;
+        Model<S>::update_aux_variables(Qp, Ap, (S)0, mesh);
+        if (s->deriv_err) return fail(s->deriv_msg);
+    }
+    (void)s;
+    (void)X;
+    (void)A;
+    return 0;
+}
 
 // Workgroups of the transport kernels for workgroup size bs and cpt cells per thread: one
 // range for a single mesh, whole workgroups per member for an ensemble (member tables).
@@ -1557,13 +1788,17 @@ void launch_prep_t(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T
     const int lim = s->opt.use_limiters, pos = s->opt.positivity;
     const T eps = s->opt.wet_dry_eps;
     if (s->opt.order < 2)
-        k_prep<MAXF, 0, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+        k_prep<MAXF, 0, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0,
+                                                           grad_buffer(s));
     else if (pos != 0)
-        k_prep<MAXF, 3, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, pos == 2);
+        k_prep<MAXF, 3, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, pos == 2,
+                                                           grad_buffer(s));
     else if (s->opt.gradient == 1)
-        k_prep<MAXF, 2, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+        k_prep<MAXF, 2, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0,
+                                                           grad_buffer(s));
     else
-        k_prep<MAXF, 1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0);
+        k_prep<MAXF, 1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0,
+                                                           grad_buffer(s));
 }
 template <int MAXF>
 void launch_prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* As, T* QF) {
@@ -1581,19 +1816,24 @@ void launch_flux_t(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T
     const int wb = s->opt.well_balanced;
     switch (mode) {
         case MODE_RHS:
-            k_flux<MAXF, MODE_RHS, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_RHS, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
+                                                                grad_buffer(s));
             break;
         case MODE_EULER:
-            k_flux<MAXF, MODE_EULER, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_EULER, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
+                                                                grad_buffer(s));
             break;
         case MODE_STAGE:
-            k_flux<MAXF, MODE_STAGE, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_STAGE, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
+                                                                grad_buffer(s));
             break;
         case MODE_IMEX3:
-            k_flux<MAXF, MODE_IMEX3, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_IMEX3, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
+                                                                grad_buffer(s));
             break;
         default:
-            k_flux<MAXF, MODE_FINAL, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0);
+            k_flux<MAXF, MODE_FINAL, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
+                                                                grad_buffer(s));
             break;
     }
 }
@@ -1629,6 +1869,17 @@ void prep_record(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* 
 void prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* QF = nullptr, T* As = nullptr) {
     QF = QF ? QF : s->QF;
     As = As ? As : s->As;
+    if (s->refresh) {
+        // FormRHS's UpdateState: the derivative aux of this stage's state first (from the
+        // unclamped state, TransportStep.hpp:202), then the per-cell clamp + aux in k_prep
+        if (hipMemcpyAsync(s->Aref, Ag, sizeof(T) * (size_t)NA * s->ld, hipMemcpyDeviceToDevice, s->stream) !=
+                hipSuccess ||
+            refresh_aux(s, Xin, s->Aref)) {
+            s->step_err = 1;
+            return;
+        }
+        Ag = s->Aref;
+    }
     if (s->maxf <= 2)
         launch_prep<2>(s, ctrl, Xin, Ag, As, QF);
     else if (s->maxf == 3)
@@ -1682,6 +1933,7 @@ void post_t(zdm_solver* s, int mode, T dt_override) {
     switch (mode) {
         case 5: k_post<5, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
         case 6: k_post<4, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
+        case 7: k_post<6, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); return;
         case 0: k_post<0, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
         case 1: k_post<1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
         case 2: k_post<2, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, s->ctrl, s->X, s->A, s->partials, dt_override); break;
@@ -1847,13 +2099,7 @@ int zdm_model_boundary_tags(char* names64, int cap) {
 
 int zdm_model_initial_condition(const double* X, const double* params, double* Q) {
     if (!X || !params || !Q) return fail("zdm_model_initial_condition: null argument");
-    if constexpr (HasInitialCondition<T>::value) {
-        auto q = M::initial_condition(X, params);
-        for (int i = 0; i < NQ; ++i) Q[i] = q[i];
-        return 0;
-    } else {
-        return fail("No io.initial_condition_file set and the model emits no initial_condition(x,p)");
-    }
+    return model_initial_condition<T>(X, params, Q);
 }
 
 int zdm_model_parameter_names(char* names64, int cap) {
@@ -1887,6 +2133,20 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
         return fail("time_integration imex / implicit: not available for an ensemble");
     if (opts->time_integration != 0 && NQ < 2)
         return fail("time_integration imex / implicit needs the [b, h, ...] state layout");
+    // RefreshDerivativeAux (solver.refresh_derivative_aux) runs for a model with the
+    // field-level update_aux_variables in the splitting step (every stage's FormRHS and every
+    // EnforcePhysicalConstraints); the fused MOOD override and the IMEX / implicit steps do
+    // not carry it
+    if (HasDiffusiveFlux<T>::value && opts->positivity != 0 && opts->order >= 2)
+        return fail("a model with diffusive_flux is not available with the positivity-preserving (eta-WB) "
+                    "reconstruction");
+    const bool refresh = HasFieldAuxUpdate<T>::value && opts->refresh_derivative_aux != 0;
+    if (refresh && opts->positivity == 1)
+        return fail("refresh_derivative_aux with a field-level update_aux_variables is not available with "
+                    "positivity \"mood\"; set solver.refresh_derivative_aux to false");
+    if (refresh && opts->time_integration != 0)
+        return fail("refresh_derivative_aux with a field-level update_aux_variables is available for the "
+                    "splitting time integration only; set solver.refresh_derivative_aux to false");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
 
@@ -2023,6 +2283,8 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     size_t oX = take(sQ), oW = take(sQ), oF = take(sQ), oA = take(sA), oAs = take(sA), oAt = take(sA);
     size_t oQF = take(sizeof(T) * (size_t)MFa * NQP * ld);
     size_t oXs = take(s->opt.positivity == 1 ? sQ : 8);   // MOOD: the saved start-of-step state
+    size_t oAref = take(refresh ? sA : 8);                  // the refreshed aux of a transport stage
+    size_t oG = take(HasDiffusiveFlux<T>::value ? sizeof(T) * (size_t)NQ * DIM * ld : 8);
     size_t oIm = take(s->opt.time_integration == 1 ? 4 * sQ : 8);   // IMEX: G, stage state, P3, F
     size_t oPart = take(sizeof(T) * ((size_t)s->maxblocks + NXCD + 1));   // + k_finalize_xcd's minima, counter
     size_t oStage = take(sizeof(T) * (size_t)std::max(NQ, NA) * ld);
@@ -2052,6 +2314,9 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     char* a = s->arena;
     s->X = (T*)(a + oX);
     s->Xs = (T*)(a + oXs);
+    s->Aref = (T*)(a + oAref);
+    s->G = (T*)(a + oG);
+    s->refresh = refresh;
     s->Im = (T*)(a + oIm);
     s->W = (T*)(a + oW);
     s->F = (T*)(a + oF);
@@ -2249,6 +2514,7 @@ int zdm_initialize(zdm_solver* s) {
     for (Ctrl& c : c0) c.active = 1;
     HIPCHK(hipMemcpyAsync(s->ctrl, c0.data(), sizeof(Ctrl) * c0.size(), hipMemcpyHostToDevice, s->stream));
     if (s->imp) s->imp->have_prev = false;   // BDF history restarts with BDF1
+    if (s->refresh && refresh_aux(s, s->X, s->A)) return -1;
     post(s, 1);
     finalize(s, 1);
     HIPCHK(hipGetLastError());
@@ -2355,10 +2621,28 @@ static void enqueue_step(zdm_solver* s) {
         flux(s, s->ctrl, MODE_FINAL, st - 1, s->W, s->X, s->A);
         record(s, 1);
     }
-    if (mood)
+    if (mood) {
         post_mood(s);
-    else
+    } else if (s->refresh) {
+        // MUSCLSolver::Run after TSStep (MUSCLSolver.hpp:153-182) with the RefreshDerivativeAux
+        // of each EnforcePhysicalConstraints: [refresh, constraints, source,] refresh,
+        // constraints, CFL
+        if (s->opt.source) {
+            if (refresh_aux(s, s->X, s->A)) {
+                s->step_err = 1;
+                return;
+            }
+            post(s, 6);
+            post(s, 7);
+        }
+        if (refresh_aux(s, s->X, s->A)) {
+            s->step_err = 1;
+            return;
+        }
+        post(s, 1);
+    } else {
         post(s, s->opt.source ? 2 : 1);
+    }
     record(s, 2);
     finalize(s, 2);
 }
@@ -2479,6 +2763,10 @@ int zdm_transport_rhs(zdm_solver* s, double t, const double* X, const double* A,
     }
     if (set_ctrl_rhs_time(s, t)) return -1;
     prep(s, s->ctrl_rhs, s->W, s->Atmp);
+    if (s->step_err) {
+        s->step_err = 0;
+        return -1;
+    }
     flux(s, s->ctrl_rhs, MODE_RHS, 0, s->W, s->F, s->Atmp);
     HIPCHK(hipGetLastError());
     return download_soa(s, s->F, F, NQ, false);
@@ -2496,6 +2784,7 @@ int zdm_source_step(zdm_solver* s, double dt) {
 int zdm_enforce_constraints(zdm_solver* s) {
     if (!s) return fail("null solver");
     if (set_ctrl_rhs_time(s, 0.0)) return -1;
+    if (s->refresh && refresh_aux(s, s->X, s->A)) return -1;
     post(s, 4);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -2784,6 +3073,10 @@ extern "C" {
 int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
                       const int* gauges, const double* obs, int exact_limiter) {
     if (!s) return fail("null solver");
+    if (HasDiffusiveFlux<T>::value)
+        return fail("the adjoint does not transpose a model's diffusive_flux");
+    if (s->refresh)
+        return fail("the adjoint does not transpose RefreshDerivativeAux: set solver.refresh_derivative_aux to false");
     if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 0)) return -1;
     s->adj->freeze_limiter = exact_limiter ? 0 : 1;
     return 0;
@@ -2792,6 +3085,10 @@ int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_e
 int zdm_adjoint_setup_chorin(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
                              const int* gauges, const double* obs, int exact_limiter) {
     if (!s) return fail("null solver");
+    if (HasDiffusiveFlux<T>::value)
+        return fail("the adjoint does not transpose a model's diffusive_flux");
+    if (s->refresh)
+        return fail("the adjoint does not transpose RefreshDerivativeAux: set solver.refresh_derivative_aux to false");
     if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 1)) return -1;
     s->adj->freeze_limiter = exact_limiter ? 0 : 1;
     return 0;

@@ -1,8 +1,8 @@
 // Forward-mode dual numbers: a value and N directional derivatives, for host and device.
 //
 // The generated-model surface (Model<T>, Numerics<T>) is written against a scalar template
-// parameter and only uses +, -, *, /, sqrt, fabs, max and min, so instantiating it on
-// Dual<N> yields the exact derivatives of the same arithmetic (branches of max / min / the
+// parameter and uses +, -, *, /, sqrt, fabs, pow, max and min, so instantiating it on Dual<N>
+// yields the exact derivatives of the same arithmetic (branches of max / min / the
 // comparisons follow the values, i.e. they are frozen at the point of evaluation).  Used for
 // per-face and per-cell Jacobian-transpose products in the discrete adjoint and for the
 // tangent-linear reference run of the oracle.
@@ -117,6 +117,36 @@ template <int N>
 ZDM_HD inline Dual<N> fabs(const Dual<N>& a) { return a.v < 0 ? -a : a; }
 template <int N>
 ZDM_HD inline bool isfinite(const Dual<N>& a) { return std::isfinite(a.v); }
+// x^y for a constant exponent: d(x^y) = y x^(y-1) dx (0 for y = 0)
+template <int N>
+ZDM_HD inline Dual<N> pow(const Dual<N>& a, double y) {
+    Dual<N> r(std::pow(a.v, y));
+    const double dy = y == 0.0 ? 0.0 : y * std::pow(a.v, y - 1.0);
+    for (int k = 0; k < N; ++k) r.d[k] = dy * a.d[k];
+    return r;
+}
+template <int N>
+ZDM_HD inline Dual<N> pow(const Dual<N>& a, int n) { return pow(a, (double)n); }
+// a^y for a dual exponent: d(a^y) = a^y ln(a) dy (+ y a^(y-1) da)
+template <int N>
+ZDM_HD inline Dual<N> pow(double a, const Dual<N>& y) {
+    const double p = std::pow(a, y.v);
+    Dual<N> r(p);
+    const double l = p * std::log(a);
+    for (int k = 0; k < N; ++k) r.d[k] = l * y.d[k];
+    return r;
+}
+template <int N>
+ZDM_HD inline Dual<N> pow(const Dual<N>& a, const Dual<N>& y) {
+    const double p = std::pow(a.v, y.v);
+    Dual<N> r(p);
+    const double da = y.v == 0.0 ? 0.0 : y.v * std::pow(a.v, y.v - 1.0);
+    bool dep = false;
+    for (int k = 0; k < N; ++k) dep |= y.d[k] != 0.0;
+    const double dl = dep ? p * std::log(a.v) : 0.0;
+    for (int k = 0; k < N; ++k) r.d[k] = da * a.d[k] + dl * y.d[k];
+    return r;
+}
 }  // namespace std
 
 // value of a scalar or of a dual number

@@ -2,7 +2,9 @@
 // (Model.H:29-398, generated by create_model.py from zoomy_core).  Restated from its
 // equations, not copied; same state, aux, parameters, tags and semantics:
 //   state  [b, h, hu]                      (Model.H:31-40)
-//   aux    [dq0dx, dhdx, dbdx, hinv]       (slots 0-2 pass through; Model.H:209-222)
+//   aux    [dq0dx, dhdx, dbdx, hinv]       (slots 0-2 pass through the pointwise update,
+//                                           Model.H:209-222, and are filled by the field-level
+//                                           update_aux_variables, Model.H:250-262)
 //   params [g, rho, nu, lambda_s, e_x] = [9.81, 1, 1, 0.5, 0]   (Model.H:43-44)
 //   source S_hu = e_x g h - hinv lambda_s hu / rho               (Model.H:110-123)
 //   clamp  |hu| <= 30 max(0, h - 0.01)                           (Model.H:195-207)
@@ -96,6 +98,15 @@ struct Model {
         const T h2 = h * h, hm2 = hm * hm;
         r[3] = (T)1.4142135623730951 * std::max((T)0, h) / std::sqrt(h2 * h2 + hm2 * hm2);
         return r;
+    }
+
+    // Field-level, mesh-aware refresh of the derivative aux (Model.H:250-262): aux slots 0-2
+    // are d(hu)/dx, dh/dx and db/dx of the per-cell fields, by compute_derivative
+    // (ZoomyMesh and compute_derivative come from UserFunctions.H, included before this header)
+    static inline void update_aux_variables(T* const* Q, T* const* Qaux, const T dt, const ZoomyMesh& mesh) {
+        compute_derivative(Qaux[0], Q[2], 1, 0, 0, mesh);
+        compute_derivative(Qaux[1], Q[1], 1, 0, 0, mesh);
+        compute_derivative(Qaux[2], Q[0], 1, 0, 0, mesh);
     }
 
     PORTABLE_FN static inline SimpleArray<T, 3> boundary_conditions(const int bc_idx, const T* Q,
