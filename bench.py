@@ -162,7 +162,7 @@ def run_c3(zdm, cases, args):
     n = args.grid if args.grid != 1000 else 707
     mesh = zdm.Mesh.tri_grid(n, n, jitter=0.15, seed=3, renumber=args.order)
     X = cases.c3_state(mesh)
-    krylov = dict(restart=30, max_it=args.max_it, rtol=1e-10, atol=1e-12, op_mode=args.op_mode)
+    krylov = dict(restart=args.restart, max_it=args.max_it, rtol=1e-10, atol=1e-12, op_mode=args.op_mode)
     s = zdm.Solver("vam", mesh, order=1, cfl=0.45, t_end=1e9, source=True)
     s.chorin_setup(**krylov)
     s.upload(X, None)
@@ -170,7 +170,19 @@ def run_c3(zdm, cases, args):
     steps_w, its_w, _ = s.chorin_step(args.warmup)
     s.synchronize()
     t0 = time.perf_counter()
-    steps, its, reason = s.chorin_step(args.steps)
+    steps, its, reason = 0, 0, 0
+    step_its, step_rel = [], []
+    for _ in range(args.steps):   # one step per call: the per-step solve statistics
+        n1, it1, reason = s.chorin_step(1)
+        if n1 == 0:
+            break
+        steps += n1
+        its += it1
+        _, _, rn, rn0 = s.chorin_stats()
+        step_its.append(it1)
+        step_rel.append(rn / rn0 if rn0 > 0 else 0.0)
+        print("c3 step %d: %d GMRES iterations, reason %d, relative residual %.3e"
+              % (steps, it1, reason, step_rel[-1]), file=sys.stderr, flush=True)
     s.synchronize()
     elapsed = time.perf_counter() - t0
     t_sim, dt, nsteps = s.time()
@@ -193,12 +205,15 @@ def run_c3(zdm, cases, args):
         "data": "synthetic",
         "config": {
             "workload": "VAM level 1 Chorin split, %dx%d grid jittered +-15%% (%d cells), "
-                        "solitary wave a=0.2 over bumps <= 0.1, GMRES(30) + PBJ rtol 1e-10"
-                        % (n, n, ncells),
+                        "solitary wave a=0.2 over bumps <= 0.1, GMRES(%d) + PBJ rtol 1e-10"
+                        % (n, n, ncells, args.restart),
             "cells": ncells,
             "op_mode": ["matrix-free residual", "assembled Pmat"][args.op_mode],
             "gmres_its_per_step": round(its / max(steps, 1), 2),
+            "gmres_its_each_step": step_its,
             "last_reason": reason,
+            "converged": reason > 0,
+            "rel_residual_each_step": [float("%.3e" % r) for r in step_rel],
             "max_it": args.max_it,
             "ms_per_gmres_it": round(elapsed * 1e3 / max(its, 1), 4),
             "mcell_its_per_s": round(ncells * its / elapsed / 1e6, 3),
@@ -535,6 +550,7 @@ def main():
                     help="c3 pressure operator: 1 the device-assembled Pmat (sliced ELL SpMV, "
                          "default), 0 the matrix-free residual")
     ap.add_argument("--max-it", type=int, default=400, help="c3 GMRES iteration cap")
+    ap.add_argument("--restart", type=int, default=30, help="c3 GMRES restart (the configuration: 30)")
     ap.add_argument("--checkpoint-every", type=int, default=0,
                     help="c5 trajectory layout: -2/-1 tape with/without face states, 1 every "
                          "state, k>1 checkpoints, -2-c binomial schedule on c state slots, 0 "

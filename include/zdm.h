@@ -177,6 +177,11 @@ int zdm_chorin_setup(zdm_solver* s, const zdm_krylov_options* opts);
  * -4 dtol, -9 NaN).  Synchronises once per GMRES iteration. */
 int zdm_chorin_step(zdm_solver* s, int n, int* steps_done, int* its_total, int* last_reason);
 
+/* The last pressure solve: GMRES iterations, KSPConvergedReason and the preconditioned
+ * residual norm at the end and at the start (rnorm / rnorm0 is the relative residual the
+ * rtol test compares; a solve stopped by max_it reports how far it got). */
+int zdm_chorin_stats(zdm_solver* s, int* its, int* reason, double* rnorm, double* rnorm0);
+
 /* The last solved pressure, NP = 2 per cell (the dmP Vec layout). */
 int zdm_chorin_pressure(zdm_solver* s, double* P);
 

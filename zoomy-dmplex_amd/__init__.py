@@ -304,7 +304,7 @@ SOLVER_SYMBOLS = [
     "zdm_transport_rhs", "zdm_source_step", "zdm_enforce_constraints", "zdm_cfl_dt",
     "zdm_state_upload_device", "zdm_state_download_device", "zdm_enable_timing",
     "zdm_kernel_time", "zdm_stream_copy", "zdm_last_error",
-    "zdm_chorin_setup", "zdm_chorin_step", "zdm_chorin_pressure", "zdm_vam_frozen_aux",
+    "zdm_chorin_setup", "zdm_chorin_step", "zdm_chorin_pressure", "zdm_chorin_stats", "zdm_vam_frozen_aux",
     "zdm_vam_residual", "zdm_vam_pmat", "zdm_bsr2_gmres", "zdm_batch_create", "zdm_batch_time",
     "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
     "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout", "zdm_troubled_cells",
@@ -379,6 +379,7 @@ def model_lib(model):
     lib.zdm_chorin_setup.argtypes = [V, K]
     lib.zdm_chorin_step.argtypes = [V, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
     lib.zdm_chorin_pressure.argtypes = [V, _c_dbl_p]
+    lib.zdm_chorin_stats.argtypes = [V, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p]
     lib.zdm_vam_frozen_aux.argtypes = [V, _c_dbl_p]
     lib.zdm_vam_residual.argtypes = [V, ctypes.c_double, _c_dbl_p, _c_dbl_p]
     lib.zdm_vam_pmat.argtypes = [V, ctypes.c_double, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p]
@@ -694,6 +695,15 @@ class Solver:
         self._check(self.lib.zdm_chorin_step(self._h, int(n), ctypes.byref(st), ctypes.byref(its),
                                              ctypes.byref(rs)))
         return st.value, its.value, rs.value
+
+    def chorin_stats(self):
+        """The last pressure solve: (iterations, KSPConvergedReason, final and initial
+        preconditioned residual norms)."""
+        it, rs = ctypes.c_int(), ctypes.c_int()
+        rn, rn0 = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.zdm_chorin_stats(self._h, ctypes.byref(it), ctypes.byref(rs), ctypes.byref(rn),
+                                              ctypes.byref(rn0)))
+        return it.value, rs.value, rn.value, rn0.value
 
     def chorin_pressure(self):
         P = np.empty((self.mesh.ncells, 2))

@@ -3000,6 +3000,15 @@ int zdm_chorin_step(zdm_solver* s, int n, int* steps_done, int* its_total, int* 
     return 0;
 }
 
+int zdm_chorin_stats(zdm_solver* s, int* its, int* reason, double* rnorm, double* rnorm0) {
+    if (!s || !s->ch) return fail("zdm_chorin_stats: no pressure solve on this solver");
+    if (its) *its = s->ch->last_its;
+    if (reason) *reason = s->ch->last_reason;
+    if (rnorm) *rnorm = s->ch->last_rnorm;
+    if (rnorm0) *rnorm0 = s->ch->last_rnorm0;
+    return 0;
+}
+
 int zdm_chorin_pressure(zdm_solver* s, double* P) {
     if (!s || !s->ch || !P) return fail("zdm_chorin_pressure: null argument");
     HIPCHK(hipMemcpyAsync(P, s->ch->P, sizeof(T) * 2 * (size_t)s->ncells, hipMemcpyDeviceToHost, s->stream));
@@ -3056,6 +3065,7 @@ static int not_vam() {
 int zdm_chorin_setup(zdm_solver*, const zdm_krylov_options*) { return not_vam(); }
 int zdm_chorin_step(zdm_solver*, int, int*, int*, int*) { return not_vam(); }
 int zdm_chorin_pressure(zdm_solver*, double*) { return not_vam(); }
+int zdm_chorin_stats(zdm_solver*, int*, int*, double*, double*) { return not_vam(); }
 int zdm_vam_frozen_aux(zdm_solver*, double*) { return not_vam(); }
 int zdm_vam_residual(zdm_solver*, double, const double*, double*) { return not_vam(); }
 int zdm_vam_pmat(zdm_solver*, double, int, int*, int*, double*) { return not_vam(); }
