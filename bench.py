@@ -81,11 +81,19 @@ import sys, time, json
 sys.path.insert(0, sys.argv[1])
 from conftest import oracle, zdm, cases
 import numpy as np
-grid, steps = int(sys.argv[2]), int(sys.argv[3])
-orc = oracle("sme2")
-mesh = zdm.Mesh.tri_grid(grid, grid)
-info = zdm.ModelInfo("sme2")
-X = cases.c2_state(mesh, info.n_q, seed=2)
+workload, arg, steps = sys.argv[2], int(sys.argv[3]), int(sys.argv[4])
+if workload == "c2":
+    model = "sme2"
+    mesh = zdm.Mesh.tri_grid(arg, arg)
+    info = zdm.ModelInfo(model)
+    X = cases.c2_state(mesh, info.n_q, seed=2)
+else:
+    model = "sme1"
+    n, mesh_seed, state = cases.ensemble_member(arg)
+    mesh = zdm.Mesh.tri_grid(n, n, jitter=0.15, seed=mesh_seed)
+    info = zdm.ModelInfo(model)
+    X = state(mesh)
+orc = oracle(model)
 A = np.zeros((mesh.ncells, info.n_aux))
 fb = zdm.boundary_face_bc(mesh, info.boundary_tags, info.dim)
 runs = []
@@ -97,35 +105,122 @@ for n in (steps, 2 * steps):
 print(json.dumps({"cells": mesh.ncells, "per_step": (runs[1] - runs[0]) / steps}))
 """
 
+# The measuring host gives each GPU a share of 16 cores, whatever os.cpu_count() reports there
+# (it counts the whole machine): the CPU baselines run at most that many processes.
+CPU_SHARE_PER_GPU = 16
 
-def cpu_baseline(zdm, cases, grid=250, steps=20, procs=None):
-    """The oracle (C++ restatement of the reference step, one thread each) run as `procs`
-    concurrent processes on all host cores (at most 16), each on the same bounded sample; the
-    aggregate throughput is reported (slope of T vs 2T steps removes start-up)."""
+
+def host_cpu():
+    """os.cpu_count() and the CPU model of this host."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"os_cpu_count": os.cpu_count(), "cpu_model": model}
+
+
+def _oracle_procs(args_list, steps):
+    """One oracle process per entry of args_list (workload, arg), all at once."""
     import subprocess
-    procs = procs or max(1, min(16, os.cpu_count() or 1))
     env = dict(os.environ, ZDM_NO_TORCH="1", OMP_NUM_THREADS="1")
-    cmd = [sys.executable, "-c", _CPU_CHILD, os.path.join(ROOT, "tests"), str(grid), str(steps)]
-    ps = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, text=True)
-          for _ in range(procs)]
+    ps = [subprocess.Popen([sys.executable, "-c", _CPU_CHILD, os.path.join(ROOT, "tests"), w, str(a), str(steps)],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, text=True)
+          for w, a in args_list]
     res = []
     for p in ps:
         out, err = p.communicate()
         if p.returncode != 0:
             raise RuntimeError("CPU baseline process failed: " + err[-2000:])
         res.append(json.loads(out.strip().splitlines()[-1]))
+    return res
+
+
+def cpu_baseline(zdm, cases, grid=250, steps=20, procs=None):
+    """The oracle (C++ restatement of the reference step, one thread each) run as `procs`
+    concurrent processes on the host's cores (at most the 16-core share of one GPU), each on
+    the same bounded sample; the aggregate throughput is reported (slope of T vs 2T steps
+    removes start-up)."""
+    procs = procs or max(1, min(CPU_SHARE_PER_GPU, os.cpu_count() or 1))
+    res = _oracle_procs([("c2", grid)] * procs, steps)
     cells = res[0]["cells"]
     per_step = sum(r["per_step"] for r in res) / len(res)
-    return {
+    return dict({
         "value": procs * cells / per_step / 1e6,
         "unit": "Mcell-steps/s",
         "cores": procs,
         "kind": "port",
         "sample": "oracle (C++ restatement of the reference step, one thread per process) as %d "
-                  "concurrent processes, each on the same workload at %dx%d (%d cells), slope of %d "
-                  "vs %d steps: %.1f ns/cell/step per process"
-                  % (procs, grid, grid, cells, steps, 2 * steps, per_step / cells * 1e9),
-    }
+                  "concurrent processes (the 16-core share of one GPU on the measuring host), each on "
+                  "the same workload at %dx%d (%d cells), slope of %d vs %d steps: %.1f ns/cell/step "
+                  "per process" % (procs, grid, grid, cells, steps, 2 * steps, per_step / cells * 1e9),
+    }, **host_cpu())
+
+
+def c4_cpu_baseline(zdm, cases, members, steps=10, procs=None):
+    """The oracle on the ensemble's own members, one member per process, `procs` processes at
+    once (members spread over the ensemble's index range): aggregate Mcell-steps/s."""
+    procs = procs or max(1, min(CPU_SHARE_PER_GPU, os.cpu_count() or 1))
+    pick = sorted(set(int(round(i * (members - 1) / max(procs - 1, 1))) for i in range(procs)))
+    res = _oracle_procs([("c4", i) for i in pick], steps)
+    value = sum(r["cells"] / r["per_step"] for r in res) / 1e6
+    return dict({
+        "value": value,
+        "unit": "Mcell-steps/s",
+        "cores": len(pick),
+        "kind": "port",
+        "sample": "oracle (C++ restatement of the reference step, one thread per process) on %d of the "
+                  "%d members (%d..%d cells), one per process, all at once (the 16-core share of one GPU), "
+                  "slope of %d vs %d steps, throughputs summed: the CPU's ensemble rate when each core "
+                  "runs its own member" % (len(pick), members, min(r["cells"] for r in res),
+                                           max(r["cells"] for r in res), steps, 2 * steps),
+    }, **host_cpu())
+
+
+def c5_cpu_baseline(zdm, cases, grid=120, steps=10):
+    """The C5 problem on the CPU: the oracle's forward loss and its tangent-linear run (one
+    direction) on a bounded sample, one thread.  value = tangent-linear time / forward time,
+    the CPU path's cost of one directional derivative (a full gradient over bed and Manning n
+    would take 2 x cells of them)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import TangentOracle, oracle  # CPU checker, timed here as the reported baseline
+    mesh = zdm.Mesh.tri_grid(grid, grid, jitter=0.15, seed=5)
+    _, _, Xg, Ag, gauges = cases.c5_case(mesh)
+    info = zdm.ModelInfo("sme1m")
+    orc, tl = oracle("sme1m"), TangentOracle("sme1m")
+    fb = zdm.boundary_face_bc(mesh, info.boundary_tags, info.dim)
+    p = info.default_params
+    dt = 0.5 * orc.compute_dt(mesh, p, Xg.copy(), Ag.copy(), 0.45)
+    db, dn = c5_directions(mesh.ncells, 1)[0]
+    dX, dA = np.zeros_like(Xg), np.zeros_like(Ag)
+    dX[:, 0], dA[:, 1] = db, dn
+    tf, tt = [], []
+    for n in (steps, 2 * steps):
+        dts = np.full(n, dt)
+        obs = np.zeros((n // 5, len(gauges)))
+        t0 = time.perf_counter()
+        orc.loss(mesh, fb, p, Xg, Ag, dts, gauges, 5, obs)
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        tl.loss(mesh, fb, p, Xg, dX, Ag, dA, dts, gauges, 5, obs)
+        tt.append(time.perf_counter() - t0)
+    fwd, tan = (tf[1] - tf[0]) / steps, (tt[1] - tt[0]) / steps
+    return dict({
+        "value": round(tan / fwd, 3),
+        "unit": "x forward time (one directional derivative)",
+        "cores": 1,
+        "kind": "port",
+        "forward_mcell_steps_per_s": round(mesh.ncells / fwd / 1e6, 3),
+        "sample": "oracle forward loss and its tangent-linear run (dual numbers, one direction in bed "
+                  "and Manning n, exact limiter), %dx%d jittered grid (%d cells), fixed dt, slope of %d vs "
+                  "%d steps, 1 thread; the CPU path has no adjoint: a full gradient takes one "
+                  "tangent-linear run per parameter (%d here)"
+                  % (grid, grid, mesh.ncells, steps, 2 * steps, 2 * mesh.ncells),
+    }, **host_cpu())
 
 
 def c3_cpu_baseline(zdm, cases, grid=60, steps=1, krylov=None):
@@ -245,7 +340,8 @@ def run_c3adj(zdm, cases, args):
     s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, chorin=True)
     s.adjoint_forward(Xt, None)
     obs = s.adjoint_residuals().copy()
-    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs, checkpoint_every=args.checkpoint_every,
+    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs,
+                    checkpoint_every=0 if args.checkpoint_every is None else args.checkpoint_every,
                     chorin=True)
     ck, _ = s.adjoint_layout()
     s.upload(Xg, None)
@@ -461,24 +557,77 @@ def run_c4(zdm, cases, args):
         },
         "roofline": None,
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = c4_cpu_baseline(zdm, cases, count)
     print(json.dumps(out))
 
 
-def run_c5(zdm, cases, args):
-    """Calibration gradient: forward run + adjoint sweep vs the plain forward run."""
+def c5_problem(zdm, cases, args):
     n = args.grid if args.grid != 1000 else 354
     mesh = zdm.Mesh.tri_grid(n, n, jitter=0.15, seed=5, renumber=args.order)
     Xt, At, Xg, Ag, gauges = cases.c5_case(mesh)
-    T = args.steps
-    obs_every = 5
     s = zdm.Solver("sme1m", mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e9, source=True,
                    use_limiters=True)
-    # synthetic observations: h at the gauges every 5 steps of the hidden-truth run
+    return n, mesh, s, Xt, At, Xg, Ag, gauges
+
+
+def c5_observations(s, T, gauges, obs_every, Xt, At):
+    """h at the gauges every obs_every steps of the hidden-truth run."""
     s.adjoint_setup(T, gauges=gauges, obs_every=obs_every)
     s.adjoint_forward(Xt, At)
-    obs = s.adjoint_residuals().copy()
-    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs, checkpoint_every=args.checkpoint_every)
-    ck, tape = s.adjoint_layout()
+    return s.adjoint_residuals().copy()
+
+
+def c5_default_layout(T):
+    """The configured layout: the binomial schedule on ceil(sqrt(T)) state slots."""
+    return -2 - int(np.ceil(np.sqrt(T)))
+
+
+def c5_measure(s, T, gauges, obs_every, obs, X, A, ck, exact, stretch=None):
+    """Recording forward + reverse sweep, once to warm up and once timed."""
+    s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs, checkpoint_every=ck,
+                    exact_limiter=exact, stretch_tape=stretch)
+    s.adjoint_forward(X, A)
+    s.adjoint_gradient()
+    s.synchronize()
+    t0 = time.perf_counter()
+    loss = s.adjoint_forward(X, A)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gX, gA = s.adjoint_gradient()
+    t_rev = time.perf_counter() - t0
+    if not (np.all(np.isfinite(gX)) and np.all(np.isfinite(gA))):
+        raise RuntimeError("non-finite gradient (checkpoint_every %d, exact %s)" % (ck, exact))
+    lay = s.adjoint_layout()
+    st = s.adjoint_stretch_tape()
+    return {"t_fwd": t_fwd, "t_rev": t_rev, "loss": loss, "gX": gX, "gA": gA, "layout": lay, "stretch": st}
+
+
+def c5_layout_name(m):
+    ck, tape = m["layout"]
+    L, lv = m["stretch"]
+    if ck < -2:
+        name = "binomial schedule on %d state slots" % (-2 - ck)
+    elif tape:
+        return ["", "tape of stage states", "tape of stage and face states"][tape]
+    elif ck == 1:
+        name = "every state"
+    else:
+        name = "checkpoint every %d steps" % ck
+    if L:
+        name += ", stretch tape of %d steps (%s)" % (L, "stage and face states" if lv == 2 else "stage states")
+    return name
+
+
+def run_c5(zdm, cases, args):
+    """Calibration gradient: forward run + adjoint sweep vs the plain forward run, on the
+    binomial schedule over ceil(sqrt(T)) state slots by default."""
+    T = args.steps
+    obs_every = 5
+    n, mesh, s, Xt, At, Xg, Ag, gauges = c5_problem(zdm, cases, args)
+    obs = c5_observations(s, T, gauges, obs_every, Xt, At)
+    ck = c5_default_layout(T) if args.checkpoint_every is None else args.checkpoint_every
+    exact = bool(args.exact_limiter)
     # plain forward of the first guess (the same steps, no recording)
     s.upload(Xg, Ag)
     s.initialize()
@@ -491,18 +640,36 @@ def run_c5(zdm, cases, args):
     s.step(T)
     s.synchronize()
     t_plain = time.perf_counter() - t0
-    s.adjoint_forward(Xg, Ag)          # warm-up of the recording forward and the sweep
-    s.adjoint_gradient()
-    t0 = time.perf_counter()
-    loss = s.adjoint_forward(Xg, Ag)
-    t_fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    gX, gA = s.adjoint_gradient()
-    t_rev = time.perf_counter() - t0
-    if not (np.all(np.isfinite(gX)) and np.all(np.isfinite(gA))):
-        raise RuntimeError("non-finite gradient")
+    print("c5: plain forward %.4f s" % t_plain, file=sys.stderr, flush=True)
+    head = c5_measure(s, T, gauges, obs_every, obs, Xg, Ag, ck, exact)
+    print("c5: %s: %.4f + %.4f s" % (c5_layout_name(head), head["t_fwd"], head["t_rev"]), file=sys.stderr, flush=True)
+    breakdown = {}
+    if not args.no_breakdown:
+        # the same gradient from other layouts: the full tape's reverse sweep runs only the
+        # transposed kernels, so the difference is the stretches' recording re-runs
+        for key, ck2, ex2, st2 in (("full_tape", -2, exact, None), ("binomial_without_stretch_tape", ck, exact, 0),
+                                   ("other_limiter_mode", ck, not exact, None)):
+            m = c5_measure(s, T, gauges, obs_every, obs, Xg, Ag, ck2, ex2, st2)
+            same = bool(np.array_equal(m["gX"], head["gX"]) and np.array_equal(m["gA"], head["gA"]))
+            breakdown[key] = {
+                "layout": c5_layout_name(m),
+                "exact_limiter": ex2,
+                "forward_recording_s": round(m["t_fwd"], 4),
+                "reverse_s": round(m["t_rev"], 4),
+                "ratio": round((m["t_fwd"] + m["t_rev"]) / t_plain, 3),
+                "gradient_bitwise_equal_to_headline": same,
+            }
+            print("c5: %s: %.4f + %.4f s" % (key, m["t_fwd"], m["t_rev"]), file=sys.stderr, flush=True)
+        ft = breakdown["full_tape"]
+        breakdown["split_of_headline"] = {
+            "plain_forward_s": round(t_plain, 4),
+            "forward_recording_s": round(head["t_fwd"], 4),
+            "transposed_kernels_s": ft["reverse_s"],
+            "stretch_re_runs_s": round(head["t_rev"] - ft["reverse_s"], 4),
+        }
+    gX, gA = head["gX"], head["gA"]
     ncells = mesh.ncells
-    ratio = (t_fwd + t_rev) / t_plain
+    ratio = (head["t_fwd"] + head["t_rev"]) / t_plain
     out = {
         "metric": "adjoint/forward ratio (SME L1 + Manning calibration, FP64)",
         "value": round(ratio, 3),
@@ -521,40 +688,124 @@ def run_c5(zdm, cases, args):
                         "steps, 32 gauges every %d steps, gradient wrt bed and Manning n"
                         % (n, n, ncells, T, obs_every),
             "cells": ncells,
+            "limiter": "exact (alpha differentiated through its active branch)" if exact else
+                       "frozen (alpha held at its forward value)",
             "forward_s": round(t_plain, 4),
-            "forward_recording_s": round(t_fwd, 4),
-            "reverse_s": round(t_rev, 4),
-            "reverse_over_forward": round(t_rev / t_plain, 3),
+            "forward_recording_s": round(head["t_fwd"], 4),
+            "reverse_s": round(head["t_rev"], 4),
+            "reverse_over_forward": round(head["t_rev"] / t_plain, 3),
             "forward_mcell_steps_per_s": round(ncells * T / t_plain / 1e6, 2),
-            "loss": loss,
+            "loss": head["loss"],
             "grad_norm_bed": float(np.linalg.norm(gX[:, 0])),
             "grad_norm_manning": float(np.linalg.norm(gA[:, 1])),
-            "checkpoint_every": ck,
-            "tape": ("binomial schedule on %d state slots" % (-2 - ck) if ck < -2 else
-                     ["states or checkpoints", "stage states", "stage and face states"][tape]),
+            "checkpoint_every": head["layout"][0],
+            "layout": c5_layout_name(head),
+            "breakdown": breakdown,
         },
         "roofline": None,
     }
     s.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = c5_cpu_baseline(zdm, cases)
     print(json.dumps(out))
+
+
+def c5_directions(ncells, k, seed=11, kind="white", mesh=None):
+    """k seeded directions in (bed, Manning n).  white: 0.01 N(0, 1) per cell on the bed and
+    0.005 N(0, 1) on n; smooth: the same amplitudes on smooth random fields (sums of random
+    Fourier modes, as the unknown fields themselves)."""
+    out = []
+    if kind == "smooth":
+        from zoomy_dmplex_amd.cases import SplitMix64, smooth_field
+        rng = SplitMix64(seed)
+        xy = mesh.cell_centroid[:, :2]
+        for _ in range(k):
+            out.append((0.01 * smooth_field(xy, rng), 0.005 * smooth_field(xy, rng)))
+        return out
+    rng = np.random.default_rng(seed)
+    for _ in range(k):
+        out.append((0.01 * rng.standard_normal(ncells), 0.005 * rng.standard_normal(ncells)))
+    return out
+
+
+def run_c5_fd(zdm, cases, args):
+    """Central differences of the device's own frozen-dt forward run against
+    <gradient, direction> on --directions random directions (white and smooth), for each FD
+    step in --fd-steps (the configuration: 1e-6), at each horizon in --horizons (steps), in
+    both limiter modes, with the configured layout."""
+    horizons = [int(v) for v in args.horizons.split(",")]
+    epss = [float(v) for v in args.fd_steps.split(",")]
+    obs_every = 5
+    n, mesh, s, Xt, At, Xg, Ag, gauges = c5_problem(zdm, cases, args)
+    dirsets = {kind: c5_directions(mesh.ncells, args.directions, kind=kind, mesh=mesh)
+               for kind in ("smooth", "white")}
+    rows = []
+    for T in horizons:
+        oe = min(obs_every, T)
+        obs = c5_observations(s, T, gauges, oe, Xt, At)
+        for exact in (True, False):
+            s.adjoint_setup(T, gauges=gauges, obs_every=oe, obs=obs, checkpoint_every=c5_default_layout(T),
+                            exact_limiter=exact)
+            loss = s.adjoint_forward(Xg, Ag)
+            dts = s.adjoint_dts().copy()
+            gX, gA = s.adjoint_gradient()
+            for kind, dirs in dirsets.items():
+                for eps in epss:
+                    errs = []
+                    for db, dn in dirs:
+                        Xp, Xm, Ap, Am = Xg.copy(), Xg.copy(), Ag.copy(), Ag.copy()
+                        Xp[:, 0] += eps * db
+                        Xm[:, 0] -= eps * db
+                        Ap[:, 1] += eps * dn
+                        Am[:, 1] -= eps * dn
+                        fd = (s.adjoint_forward(Xp, Ap, dts=dts) - s.adjoint_forward(Xm, Am, dts=dts)) / (2 * eps)
+                        ad = float(np.dot(gX[:, 0], db) + np.dot(gA[:, 1], dn))
+                        errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
+                    errs = np.array(errs)
+                    row = {"steps": T, "exact_limiter": exact, "directions": kind, "fd_step": eps, "loss": loss,
+                           "grad_norm": float(np.sqrt(np.sum(gX[:, 0] ** 2) + np.sum(gA[:, 1] ** 2))),
+                           "rel_err_median": float(np.median(errs)), "rel_err_max": float(np.max(errs)),
+                           "within_1e-6": int(np.sum(errs <= 1e-6)), "count": len(errs)}
+                    rows.append(row)
+                    print("c5fd:", json.dumps(row), file=sys.stderr, flush=True)
+    s.close()
+    print(json.dumps({
+        "metric": "adjoint vs central differences (SME L1 + Manning calibration, FP64)",
+        "value": None,
+        "unit": "relative error per row",
+        "n_gpus": 1, "steps": horizons[-1] if horizons else 0, "warmup": 0, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+        "config": {"workload": "C5 calibration problem on %d cells, central differences on %d seeded "
+                               "directions per kind in (bed, Manning n), dt frozen at the forward run"
+                               % (mesh.ncells, args.directions),
+                   "rows": rows},
+    }))
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c3adj"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5fd", "c3adj"],
                     help="c2: SME level 2 transport + source (headline); c3: VAM Chorin split; "
                          "c4: batched vs serial ensemble; c5: adjoint calibration gradient; "
-                         "c3adj: adjoint through the Chorin split")
+                         "c5fd: its central-difference check; c3adj: adjoint through the Chorin split")
     ap.add_argument("--members", type=int, default=256, help="c4 ensemble size")
     ap.add_argument("--op-mode", type=int, default=1, choices=[0, 1],
                     help="c3 pressure operator: 1 the device-assembled Pmat (sliced ELL SpMV, "
                          "default), 0 the matrix-free residual")
     ap.add_argument("--max-it", type=int, default=400, help="c3 GMRES iteration cap")
     ap.add_argument("--restart", type=int, default=30, help="c3 GMRES restart (the configuration: 30)")
-    ap.add_argument("--checkpoint-every", type=int, default=0,
+    ap.add_argument("--checkpoint-every", type=int, default=None,
                     help="c5 trajectory layout: -2/-1 tape with/without face states, 1 every "
                          "state, k>1 checkpoints, -2-c binomial schedule on c state slots, 0 "
-                         "the richest that fits")
+                         "the richest that fits; default the binomial schedule on ceil(sqrt(steps)) "
+                         "slots")
+    ap.add_argument("--exact-limiter", action="store_true",
+                    help="c5: differentiate the limiter through its active branch (default: frozen)")
+    ap.add_argument("--no-breakdown", action="store_true",
+                    help="c5: skip the comparison layouts (full tape, no stretch tape, other limiter mode)")
+    ap.add_argument("--horizons", default="1,2,5,10,25,50,100,200,500", help="c5fd: trajectory lengths")
+    ap.add_argument("--fd-steps", default="1e-6", help="c5fd: central-difference steps")
+    ap.add_argument("--directions", type=int, default=64, help="c5fd: random directions")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: c2 200, c3 10, c4 300, c5 500, c3adj 10)")
@@ -578,11 +829,13 @@ def main():
         sys.exit(2)
 
     if args.steps is None:
-        args.steps = {"c2": 200, "c3": 10, "c4": 300, "c5": 500, "c3adj": 10}[args.workload]
+        args.steps = {"c2": 200, "c3": 10, "c4": 300, "c5": 500, "c5fd": 0, "c3adj": 10}[args.workload]
     zdm = load_package()
     import zoomy_dmplex_amd.cases as cases
     if args.workload == "c5":
         return run_c5(zdm, cases, args)
+    if args.workload == "c5fd":
+        return run_c5_fd(zdm, cases, args)
     if args.workload == "c3":
         return run_c3(zdm, cases, args)
     if args.workload == "c3adj":

@@ -240,6 +240,7 @@ int zdm_bsr2_gmres_noalloc(int nb, const int* rowptr, const int* col, const doub
  *    k  a checkpoint every k steps (each segment is re-run once more);
  *  -2-c (c >= 1) c state slots, checkpoints placed by the optimal binomial schedule
  *       (Griewank-Walther "revolve": the fewest forward re-steps for c slots);
+ *       both with a stretch tape when it fits (zdm_adjoint_stretch_tape);
  *    0  the first of -2, -1 that fits in half of the free device memory, else 1 if the
  *       states fit in a quarter of it, else the binomial schedule on the slots that do.
  * Every layout gives bitwise the same gradient.  Order 1, or order 2 with
@@ -279,9 +280,20 @@ int zdm_autotune(zdm_solver* s, int steps, int* block_size, int* cells_per_threa
  * over ensemble members; reset by zdm_initialize.  The reference prints this count per step. */
 int zdm_troubled_cells(zdm_solver* s, long long* count);
 /* The layout zdm_adjoint_setup chose: checkpoint_every in its own encoding (1 with a tape,
- * -2-c for the binomial schedule on c slots) and tape level (0 none, 1 stage states, 2 stage
- * and face states). */
+ * -2-c for the binomial schedule on c slots) and the level of a full tape (0 none, 1 stage
+ * states, 2 stage and face states; a stretch tape is reported by zdm_adjoint_stretch_tape). */
 int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape);
+/* Stretch tape of the checkpointed layouts (checkpoint_every >= 2 or <= -3): a stretch of at
+ * most `steps` steps (one segment for checkpoint_every >= 2) is re-run once in the reverse
+ * sweep with its stage states, post-stage states and (level 2) face states recorded in place,
+ * and its adjoints then run only transposed kernels, instead of each reversed step re-running
+ * its own step; the forward run records the last stretch itself.  zdm_adjoint_setup picks
+ * one: ceil(nsteps / c) steps on c binomial slots (one segment for checkpoint_every >= 2), at
+ * level 2 if it fits in half of the free device memory, else level 1, else none.  steps >= 0
+ * replaces it (0: none), re-planning the schedule; steps < 0 only queries.  steps_out /
+ * level_out (may be NULL) receive its length and level (0: none).  The gradient is bitwise
+ * the same with or without it.  Not for the Chorin split.  No counterpart in the reference. */
+int zdm_adjoint_stretch_tape(zdm_solver* s, int steps, int* steps_out, int* level_out);
 /* Forward run (X0, A0 in the zdm_state_upload layout, A0 may be NULL).  dts = NULL takes
  * dt = CFL each step and records it; otherwise the given sequence is used (a frozen-dt
  * re-run, e.g. for finite differences).  Returns the loss. */
@@ -290,6 +302,16 @@ int zdm_adjoint_forward(zdm_solver* s, const double* X0, const double* A0, const
 /* Gradient of the loss of the last forward run: gX0 (ncells*n_q), gA0 (ncells*n_aux);
  * either may be NULL. */
 int zdm_adjoint_gradient(zdm_solver* s, double* gX0, double* gA0);
+/* Device-buffer variants (AoS float64 device buffers in the zdm_state_upload layout, read and
+ * written in order on the solver stream; dts and loss stay host memory).  The forward returns
+ * after the loss is known; the gradient returns with its copies enqueued (order later work
+ * on the solver stream, or synchronise it). */
+int zdm_adjoint_forward_device(zdm_solver* s, const double* dX0, const double* dA0, const double* dts,
+                               double* loss);
+int zdm_adjoint_gradient_device(zdm_solver* s, double* dgX0, double* dgA0);
+/* The solver's stream (a non-blocking hipStream_t of default priority) for ordering
+ * a caller's work against the solver's, e.g. through events. */
+int zdm_solver_stream(zdm_solver* s, void** stream);
 /* The dt sequence (nsteps) and the residuals h - obs (nobs*ngauge) of the last forward run. */
 int zdm_adjoint_dts(zdm_solver* s, double* dts);
 int zdm_adjoint_residuals(zdm_solver* s, double* r);

@@ -1027,6 +1027,8 @@ namespace {
 //   loss = sum over steps k*obs_every (k >= 1) and gauges g of (h_g - obs[k-1][g])^2.
 // Run on double (orc_loss) and on one-direction dual numbers (orc_tl_loss, the
 // tangent-linear reference for the discrete adjoint).
+bool g_loss_source = true;   // the solver's `source` setting (false: no SourceStep)
+
 T run_loss(const zdm_mesh* m, const int* face_bc, const T* P, int order, int ssp_stages,
            int use_limiters, int nsteps, const double* dts, T* X, T* A, int ngauge,
            const int* gauges, int obs_every, const double* obs) {
@@ -1040,8 +1042,10 @@ T run_loss(const zdm_mesh* m, const int* face_bc, const T* P, int order, int ssp
         const T dt = T(dts[n]);
         ts_step(m, face_bc, P, o, T(t), dt, X, A, w);
         enforce(nc, P, X, A);
-        source_solve(nc, P, dt, X, A);
-        enforce(nc, P, X, A);
+        if (g_loss_source) {
+            source_solve(nc, P, dt, X, A);
+            enforce(nc, P, X, A);
+        }
         t += dts[n];
         if (obs_every > 0 && (n + 1) % obs_every == 0) {
             const int k = (n + 1) / obs_every - 1;
@@ -1354,6 +1358,9 @@ int orc_run(const zdm_mesh* m, const int* face_bc, const double* p, int order, i
 }  // extern "C"
 
 #endif  // ORACLE_TANGENT
+
+// the loss runs with or without the SourceStep (both builds)
+extern "C" void orc_set_loss_source(int on) { g_loss_source = on != 0; }
 
 #ifdef ORACLE_TANGENT
 // Tangent-linear run: the loss and its directional derivative along (dX0, dA0).

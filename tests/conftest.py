@@ -303,18 +303,25 @@ class TangentOracle:
         L.orc_tl_loss.argtypes = [ctypes.POINTER(zdm._ZdmMesh), _ip, _dp, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
                                   _ip, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_set_loss_source.argtypes = [ctypes.c_int]
+        L.orc_set_loss_source.restype = None
         self.L = L
 
     def loss(self, mesh, face_bc, p, X0, dX0, A0, dA0, dts, gauges, obs_every, obs, order=2,
-             ssp_stages=2, use_limiters=1, freeze_limiter=False):
+             ssp_stages=2, use_limiters=1, freeze_limiter=False, source=True):
         """(loss, directional derivative along (dX0, dA0)); freeze_limiter holds each face's
-        limiter value fixed (the frozen-limiter linearisation)."""
+        limiter value fixed (the frozen-limiter linearisation); source=False drops the
+        SourceStep (the solver's source setting)."""
         g = np.ascontiguousarray(gauges, dtype=np.int32)
         out, dout = np.zeros(1), np.zeros(1)
         c = lambda a: dp(np.ascontiguousarray(a, dtype=np.float64)) if a is not None else None
-        self.L.orc_tl_loss(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters, len(dts),
-                           c(dts), c(X0), c(dX0), c(A0), c(dA0), len(g), ip(g), obs_every, c(obs),
-                           int(freeze_limiter), dp(out), dp(dout))
+        self.L.orc_set_loss_source(int(bool(source)))
+        try:
+            self.L.orc_tl_loss(mesh.handle, ip(face_bc), dp(p), order, ssp_stages, use_limiters, len(dts),
+                               c(dts), c(X0), c(dX0), c(A0), c(dA0), len(g), ip(g), obs_every, c(obs),
+                               int(freeze_limiter), dp(out), dp(dout))
+        finally:
+            self.L.orc_set_loss_source(1)
         return out[0], dout[0]
 
     def chorin_loss(self, mesh, face_bc, p, X0, dX0, A0, dA0, dts, gauges, obs_every, obs, order=1,

@@ -21,4 +21,4 @@ def test_binomial_schedule(tmp_path):
                     "-o", exe], check=True, capture_output=True, text=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.strip().splitlines()[-1] == "cases 960 bad 0", r.stdout
+    assert r.stdout.strip().splitlines()[-1] == "cases 1590 bad 0", r.stdout

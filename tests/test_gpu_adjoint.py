@@ -77,18 +77,41 @@ def test_adjoint_vs_central_differences_on_device():
     s.close()
 
 
+def make_layout(order, steps, ck, stretch, exact):
+    mesh, X, A, bc, gauges, dts, obs = calib_case(n=6, steps=steps)
+    s = zdm.Solver(MODEL, mesh, order=order, ssp_stages=2, cfl=0.45, t_end=1e9, use_limiters=True)
+    s.adjoint_setup(steps, gauges=gauges, obs_every=2, obs=obs, checkpoint_every=ck, exact_limiter=exact,
+                    stretch_tape=stretch)
+    return s, X, A
+
+
+# (checkpoint_every, stretch tape: None automatic / 0 none / L, expected layout, expected
+# stretch tape (steps, level)); 8 steps
+LAYOUTS_8 = [(-2, None, (1, 2), (0, 0)), (-1, None, (1, 1), (0, 0)), (0, None, None, None),
+             (1, None, (1, 0), (0, 0)), (3, None, (3, 0), (3, 2)), (3, 0, (3, 0), (0, 0)),
+             (8, None, (8, 0), (8, 2)), (8, 0, (8, 0), (0, 0)),
+             (-3, None, (-3, 0), (8, 2)), (-3, 0, (-3, 0), (0, 0)), (-3, 3, (-3, 0), (3, 2)),
+             (-4, None, (-4, 0), (4, 2)), (-4, 0, (-4, 0), (0, 0)), (-4, 1, (-4, 0), (1, 2)),
+             (-5, None, (-5, 0), (3, 2)), (-5, 0, (-5, 0), (0, 0)), (-5, 2, (-5, 0), (2, 2)),
+             (-7, None, (-7, 0), (2, 2)), (-7, 0, (-7, 0), (0, 0))]
+
+
+@pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("order", [1, 2])
-def test_trajectory_layout_does_not_change_the_gradient(order):
+def test_trajectory_layout_does_not_change_the_gradient(order, exact):
     """Tape with face states (-2), tape of stage states (-1), every state (1), checkpoints
-    (3, 8), the binomial schedule on 1, 2, 3 and 5 slots (-3, -4, -5, -7) and the automatic
+    (3, 8), the binomial schedule on 1, 2, 3 and 5 slots (-3, -4, -5, -7), each with its
+    automatic stretch tape, without one and with one of another length, and the automatic
     choice (0): bitwise the same gradient, since what a tape keeps is what the re-runs
-    recompute."""
+    recompute.  In the frozen-limiter mode (exact False) the full tape with face states
+    carries the stored limiter values and the other layouts recompute them."""
     grads = []
-    for ck, want in ((-2, (1, 2)), (-1, (1, 1)), (0, None), (1, (1, 0)), (3, (3, 0)), (8, (8, 0)),
-                     (-3, (-3, 0)), (-4, (-4, 0)), (-5, (-5, 0)), (-7, (-7, 0))):
-        s, mesh, X, A, bc, gauges, obs = make(order, ck=ck)
+    for ck, stretch, want, want_st in LAYOUTS_8:
+        print("case order", order, "exact", exact, "checkpoint_every", ck, "stretch", stretch, flush=True)
+        s, X, A = make_layout(order, 8, ck, stretch, exact)
         if want is not None:
             assert s.adjoint_layout() == want
+            assert s.adjoint_stretch_tape() == want_st
         s.adjoint_forward(X, A)
         grads.append(s.adjoint_gradient())
         s.close()
@@ -97,17 +120,32 @@ def test_trajectory_layout_does_not_change_the_gradient(order):
         assert np.array_equal(g[0], grads[0][0]) and np.array_equal(g[1], grads[0][1])
 
 
-def test_binomial_schedule_on_a_longer_trajectory():
-    """21 steps on 3 and 4 slots (split by the dynamic programme, nested several levels deep):
-    bitwise the gradient of the tape."""
+@pytest.mark.parametrize("exact", [True, False])
+def test_binomial_schedule_on_a_longer_trajectory(exact):
+    """21 steps on 3 and 4 slots (split by the dynamic programme, nested several levels deep),
+    with and without stretch tapes (2 and 3 steps leave the slots too few to cover the
+    trajectory, so stretches are reversed in chunks from the scratch state), and checkpoints
+    every 4 with a stretch tape: bitwise the gradient of the tape."""
     grads = []
-    for ck in (-2, -5, -6):
-        s, mesh, X, A, bc, gauges, obs = make(2, steps=21, ck=ck)
+    for ck, stretch in ((-2, None), (-5, 0), (-5, None), (-5, 2), (-6, 0), (-6, None), (-6, 3), (4, None)):
+        print("case exact", exact, "checkpoint_every", ck, "stretch", stretch, flush=True)
+        s, X, A = make_layout(2, 21, ck, stretch, exact)
         s.adjoint_forward(X, A)
         grads.append(s.adjoint_gradient())
         s.close()
     for g in grads[1:]:
         assert np.array_equal(g[0], grads[0][0]) and np.array_equal(g[1], grads[0][1])
+
+
+def test_stretch_tape_is_refused_where_it_does_not_apply():
+    s, X, A = make_layout(2, 8, 1, None, True)
+    with pytest.raises(zdm.ZdmError, match="stretch"):
+        s.adjoint_stretch_tape(3)
+    s.close()
+    s, X, A = make_layout(2, 8, -2, None, True)
+    with pytest.raises(zdm.ZdmError, match="full tape"):
+        s.adjoint_stretch_tape(3)
+    s.close()
 
 
 def test_torch_autograd_function():
@@ -137,6 +175,43 @@ def test_torch_autograd_function():
         fd = (ag.calibration_loss(s, Xp, Ap, dts) - ag.calibration_loss(s, Xm, Am, dts)) / (2 * eps)
     ad = float(torch.sum(X0.grad[:, 0] * db) + torch.sum(A0.grad[:, 1] * dn)) / 3.0
     assert abs(ad - float(fd)) <= 1e-6 * abs(float(fd))
+    s.close()
+
+
+def test_device_entry_points_and_autograd_streams():
+    """zdm_adjoint_forward_device / zdm_adjoint_gradient_device on torch device buffers give
+    bitwise the host entry points' loss and gradient; the autograd Function gives them too when
+    torch works on a side stream (ordered through the solver stream) and from CPU tensors."""
+    torch = pytest.importorskip("torch")
+    import importlib
+    ag = importlib.import_module("zoomy_dmplex_amd.autograd")
+    s, mesh, X, A, bc, gauges, obs = make(2)
+    L_h = s.adjoint_forward(X, A)
+    gX, gA = s.adjoint_gradient()
+    Xd = torch.tensor(X, dtype=torch.float64, device="cuda")
+    Ad = torch.tensor(A, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    assert s.adjoint_forward_device(Xd.data_ptr(), Ad.data_ptr()) == L_h
+    gXd = torch.full_like(Xd, float("nan"))
+    gAd = torch.full_like(Ad, float("nan"))
+    torch.cuda.synchronize()
+    s.adjoint_gradient_device(gXd.data_ptr(), gAd.data_ptr())
+    s.synchronize()
+    assert np.array_equal(gXd.cpu().numpy(), gX) and np.array_equal(gAd.cpu().numpy(), gA)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        X0 = torch.tensor(X, dtype=torch.float64, device="cuda").requires_grad_(True)
+        A0 = torch.tensor(A, dtype=torch.float64, device="cuda").requires_grad_(True)
+        L = ag.calibration_loss(s, X0, A0)
+        (2.0 * L).backward()
+        gx2, ga2 = (X0.grad * 0.5).cpu().numpy(), (A0.grad * 0.5).cpu().numpy()
+    side.synchronize()
+    assert float(L) == L_h
+    assert np.array_equal(gx2, gX) and np.array_equal(ga2, gA)
+    X1 = torch.tensor(X, dtype=torch.float64, requires_grad=True)
+    A1 = torch.tensor(A, dtype=torch.float64, requires_grad=True)
+    ag.calibration_loss(s, X1, A1).backward()
+    assert np.array_equal(X1.grad.numpy(), gX) and np.array_equal(A1.grad.numpy(), gA)
     s.close()
 
 

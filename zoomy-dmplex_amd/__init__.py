@@ -309,7 +309,8 @@ SOLVER_SYMBOLS = [
     "zdm_step_graph", "zdm_adjoint_setup", "zdm_adjoint_forward", "zdm_adjoint_gradient",
     "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout", "zdm_troubled_cells",
     "zdm_set_launch", "zdm_get_launch", "zdm_autotune", "zdm_adjoint_setup_chorin",
-    "zdm_adjoint_krylov_its", "zdm_implicit_setup", "zdm_implicit_stats",
+    "zdm_adjoint_krylov_its", "zdm_implicit_setup", "zdm_implicit_stats", "zdm_adjoint_stretch_tape",
+    "zdm_adjoint_forward_device", "zdm_adjoint_gradient_device", "zdm_solver_stream",
 ]
 
 
@@ -370,6 +371,11 @@ def model_lib(model):
     lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_residuals.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_layout.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.zdm_adjoint_stretch_tape.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]
+    lib.zdm_adjoint_forward_device.argtypes = [V, V, V, _c_dbl_p, ctypes.POINTER(ctypes.c_double)]
+    lib.zdm_adjoint_gradient_device.argtypes = [V, V, V]
+    lib.zdm_solver_stream.argtypes = [V, ctypes.POINTER(ctypes.c_void_p)]
     lib.zdm_troubled_cells.argtypes = [V, ctypes.POINTER(ctypes.c_longlong)]
     lib.zdm_set_launch.argtypes = [V, ctypes.c_int, ctypes.c_int]
     lib.zdm_get_launch.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -577,7 +583,7 @@ class Solver:
 
     # ---- discrete adjoint (calibration gradients)
     def adjoint_setup(self, nsteps, gauges=(), obs_every=0, obs=None, checkpoint_every=0,
-                      exact_limiter=False, chorin=False):
+                      exact_limiter=False, chorin=False, stretch_tape=None):
         """Loss = sum over observation steps k*obs_every and gauges of (h - obs)^2.
         chorin=True (VAM models): the steps are Chorin pressure-split steps with the Krylov
         settings of chorin_setup; no tape layouts.
@@ -585,7 +591,9 @@ class Solver:
         linearisation); True differentiates it through its active branch.
         checkpoint_every: -2 / -1 tape with / without face states, 1 every state, k > 1 a
         checkpoint every k steps, -2-c (c >= 1) the binomial schedule on c state slots, 0 the
-        richest layout that fits in device memory."""
+        richest layout that fits in device memory.
+        stretch_tape (checkpoint_every >= 2 or <= -3): None the automatic choice, 0 none, L a
+        tape of L steps (one segment for checkpoint_every >= 2), see adjoint_stretch_tape."""
         g = np.ascontiguousarray(gauges, dtype=np.int32)
         nobs = nsteps // obs_every if len(g) and obs_every else 0
         o = np.zeros((nobs, len(g))) if obs is None else np.ascontiguousarray(obs, dtype=np.float64)
@@ -595,6 +603,16 @@ class Solver:
         fn = self.lib.zdm_adjoint_setup_chorin if chorin else self.lib.zdm_adjoint_setup
         self._check(fn(self._h, int(nsteps), int(checkpoint_every), int(obs_every), len(g),
                        g.ctypes.data_as(_c_int_p), _dp(o) if o.size else None, int(bool(exact_limiter))))
+        if stretch_tape is not None:
+            self.adjoint_stretch_tape(int(stretch_tape))
+
+    def adjoint_stretch_tape(self, steps=-1):
+        """Stretch tape of a checkpointed layout: each stretch of up to `steps` steps is re-run
+        once with its stages recorded, then reversed with transposed kernels only.  steps >= 0
+        sets it (0: none), -1 only queries; returns (steps, level), (0, 0) for none."""
+        n, lv = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.zdm_adjoint_stretch_tape(self._h, int(steps), ctypes.byref(n), ctypes.byref(lv)))
+        return n.value, lv.value
 
     def implicit_setup(self, **kw):
         """Newton / Krylov settings of time_integration "implicit" (NewtonOptions fields)."""
@@ -670,6 +688,34 @@ class Solver:
         gA = np.empty((self.mesh.ncells, self.info.n_aux))
         self._check(self.lib.zdm_adjoint_gradient(self._h, _dp(gX), _dp(gA)))
         return gX, gA
+
+    def adjoint_forward_device(self, dX0, dA0=None, dts=None):
+        """adjoint_forward on device buffers: dX0 / dA0 are device addresses (ints) of
+        (ncells, n_q) / (ncells, n_aux) float64 arrays, read on the solver stream; returns the
+        loss once it is known."""
+        dp_ = None
+        if dts is not None:
+            dts = np.ascontiguousarray(dts, dtype=np.float64)
+            if dts.size != self._adj[0]:
+                raise ZdmError("adjoint_forward_device: dts must hold nsteps values")
+            dp_ = _dp(dts)
+        L = ctypes.c_double()
+        self._check(self.lib.zdm_adjoint_forward_device(self._h, ctypes.c_void_p(dX0),
+                                                        ctypes.c_void_p(dA0) if dA0 else None, dp_,
+                                                        ctypes.byref(L)))
+        return L.value
+
+    def adjoint_gradient_device(self, dgX0, dgA0=None):
+        """adjoint_gradient into device buffers (addresses as ints, either may be 0), written
+        on the solver stream; returns with the copies enqueued."""
+        self._check(self.lib.zdm_adjoint_gradient_device(self._h, ctypes.c_void_p(dgX0) if dgX0 else None,
+                                                         ctypes.c_void_p(dgA0) if dgA0 else None))
+
+    def stream_handle(self):
+        """The solver's hipStream_t as an int (for torch.cuda.ExternalStream)."""
+        p = ctypes.c_void_p()
+        self._check(self.lib.zdm_solver_stream(self._h, ctypes.byref(p)))
+        return p.value or 0
 
     def adjoint_dts(self):
         d = np.empty(self._adj[0])

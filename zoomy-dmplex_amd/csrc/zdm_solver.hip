@@ -3087,9 +3087,7 @@ int zdm_adjoint_setup(zdm_solver* s, int nsteps, int checkpoint_every, int obs_e
         return fail("the adjoint does not transpose a model's diffusive_flux");
     if (s->refresh)
         return fail("the adjoint does not transpose RefreshDerivativeAux: set solver.refresh_derivative_aux to false");
-    if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 0)) return -1;
-    s->adj->freeze_limiter = exact_limiter ? 0 : 1;
-    return 0;
+    return adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 0, exact_limiter ? 0 : 1);
 }
 
 int zdm_adjoint_setup_chorin(zdm_solver* s, int nsteps, int checkpoint_every, int obs_every, int ngauge,
@@ -3099,9 +3097,7 @@ int zdm_adjoint_setup_chorin(zdm_solver* s, int nsteps, int checkpoint_every, in
         return fail("the adjoint does not transpose a model's diffusive_flux");
     if (s->refresh)
         return fail("the adjoint does not transpose RefreshDerivativeAux: set solver.refresh_derivative_aux to false");
-    if (adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 1)) return -1;
-    s->adj->freeze_limiter = exact_limiter ? 0 : 1;
-    return 0;
+    return adjoint_alloc(s, nsteps, checkpoint_every, obs_every, ngauge, gauges, obs, 1, exact_limiter ? 0 : 1);
 }
 
 int zdm_implicit_setup(zdm_solver* s, const zdm_newton_options* opts) {
@@ -3255,7 +3251,19 @@ int zdm_troubled_cells(zdm_solver* s, long long* count) {
 int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape) {
     if (!s || !s->adj) return fail("zdm_adjoint_layout: call zdm_adjoint_setup first");
     if (checkpoint_every) *checkpoint_every = s->adj->slots > 0 ? -2 - s->adj->slots : s->adj->ck;
-    if (tape) *tape = s->adj->tape;
+    if (tape) *tape = s->adj->stretch ? 0 : s->adj->tape;
+    return 0;
+}
+
+int zdm_adjoint_stretch_tape(zdm_solver* s, int steps, int* steps_out, int* level_out) {
+    if (!s || !s->adj) return fail("zdm_adjoint_stretch_tape: call zdm_adjoint_setup first");
+    AdjointHost* ad = s->adj;
+    if (steps >= 0) {
+        if (ad->tape && !ad->stretch) return fail("zdm_adjoint_stretch_tape: the layout is a full tape");
+        if (stretch_setup(s, ad, steps, false)) return -1;
+    }
+    if (steps_out) *steps_out = ad->stretch ? ad->tlen : 0;
+    if (level_out) *level_out = ad->stretch ? ad->tape : 0;
     return 0;
 }
 
@@ -3269,6 +3277,25 @@ int zdm_adjoint_gradient(zdm_solver* s, double* gX0, double* gA0) {
     if (!s) return fail("null solver");
     if (!s->adj) return fail("zdm_adjoint_gradient: call zdm_adjoint_setup first");
     return adjoint_reverse(s, gX0, gA0);
+}
+
+int zdm_adjoint_forward_device(zdm_solver* s, const double* dX0, const double* dA0, const double* dts,
+                               double* loss) {
+    if (!s || !dX0) return fail("zdm_adjoint_forward_device: null argument");
+    if (!s->adj) return fail("zdm_adjoint_forward_device: call zdm_adjoint_setup first");
+    return adjoint_forward(s, dX0, dA0, dts, loss, true);
+}
+
+int zdm_adjoint_gradient_device(zdm_solver* s, double* dgX0, double* dgA0) {
+    if (!s) return fail("null solver");
+    if (!s->adj) return fail("zdm_adjoint_gradient_device: call zdm_adjoint_setup first");
+    return adjoint_reverse(s, dgX0, dgA0, true);
+}
+
+int zdm_solver_stream(zdm_solver* s, void** stream) {
+    if (!s || !stream) return fail("zdm_solver_stream: null argument");
+    *stream = (void*)s->stream;
+    return 0;
 }
 
 int zdm_adjoint_dts(zdm_solver* s, double* dts) {
