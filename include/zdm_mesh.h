@@ -90,6 +90,18 @@ int zdm_mesh_concat(int n, const zdm_mesh* const* meshes, int* cell_offsets, zdm
 void zdm_mesh_destroy(zdm_mesh* m);
 const char* zdm_mesh_last_error(void);
 
+/* Initial-condition files (IOManager::LoadSolution, IOManager.hpp:122-192, reads the "state"
+ * Vec with PETSc's HDF5 viewer: (ncells, n_dof) float64 in the mesh file's cell order, or flat).
+ * libhdf5 (>= 1.10) is opened at run time: $ZDM_HDF5_LIB alone when set, else libhdf5.so,
+ * libhdf5.so.103, .200, .310, /opt/conda/lib/libhdf5.so; without it both calls fail with a
+ * message naming what was tried.
+ * zdm_h5_read: the dataset's rank (<= 4) and dims; with data non-NULL also its values,
+ * converted to float64 (capacity = room in data, in values). */
+int zdm_h5_read(const char* path, const char* dataset, double* data, uint64_t capacity, int* ndims,
+                uint64_t* dims);
+/* zdm_h5_write: a new file (an existing one is truncated) holding one float64 dataset. */
+int zdm_h5_write(const char* path, const char* dataset, const double* data, int ndims, const uint64_t* dims);
+
 /* SplitMix64: next value of the stream and a uniform double in [0,1). */
 uint64_t zdm_splitmix64_next(uint64_t* state);
 double   zdm_splitmix64_uniform(uint64_t* state);

@@ -163,6 +163,24 @@ private:
     zdm_mesh* m_ = nullptr;
 };
 
+// ------------------------------------------------------- HDF5 initial conditions (float64)
+// The "state" dataset (IOManager.hpp:122-192): ncells * nq values, flat or (ncells, nq).
+inline ErrorCode ReadH5State(const std::string& path, std::vector<double>& data, int ncells, int nq) {
+    if (!std::ifstream(path).good()) return Fail("initial condition file " + path + " cannot be opened");
+    int nd = 0;
+    uint64_t dims[4] = {0, 0, 0, 0};
+    if (zdm_h5_read(path.c_str(), "state", nullptr, 0, &nd, dims)) return Fail(zdm_mesh_last_error());
+    uint64_t n = 1;
+    for (int i = 0; i < nd; ++i) n *= dims[i];
+    const uint64_t want = (uint64_t)ncells * (uint64_t)nq;
+    if (n != want || (nd == 2 && dims[1] != (uint64_t)nq) || nd > 2)
+        return Fail(path + ": dataset \"state\" has " + std::to_string(n) + " values; the mesh and model need " +
+                    std::to_string(ncells) + " x " + std::to_string(nq));
+    data.resize(n);
+    if (zdm_h5_read(path.c_str(), "state", data.data(), n, &nd, dims)) return Fail(zdm_mesh_last_error());
+    return 0;
+}
+
 // ---------------------------------------------------------------- .npy arrays (float64)
 inline ErrorCode ReadNpy(const std::string& path, std::vector<double>& data, std::vector<size_t>& shape) {
     std::ifstream f(path, std::ios::binary);
@@ -402,9 +420,14 @@ public:
         return mesh.BoundaryMap(info.boundary_tags, face_bc);
     }
 
-    // VirtualSolver::SetupInitialConditions (VirtualSolver.hpp:234-281): the IC file when set
-    // (here: a float64 .npy of shape (ncells, k) in the mesh file's cell order, fields by
-    // initial_condition_mask when given), else Model::initial_condition at the centroids.
+    // VirtualSolver::SetupInitialConditions (VirtualSolver.hpp:234-281): the IC file when set,
+    // else Model::initial_condition at the centroids.  The file is read as
+    // IOManager::LoadSolution reads it (IOManager.hpp:122-192): an HDF5 dataset "state" of
+    // ncells * n_q float64 values, cell-major in the mesh file's cell order (flat, or
+    // (ncells, n_q)), mapped to the solver's cell order by cell_perm; without a mask every
+    // component is taken, with initial_condition_mask only the listed components (the others
+    // stay 0).  A float64 .npy of shape (ncells, k) is read the same way when k = n_q; with
+    // k < n_q its columns are the mask's components in order.
     virtual ErrorCode SetupInitialConditions() {
         const zdm_mesh* m = mesh.get();
         const int nc = m->ncells, nq = info.n_q;
@@ -412,21 +435,33 @@ public:
         A.assign((size_t)nc * info.n_aux, 0.0);
         const std::string& icf = settings.io.initial_condition_file;
         if (!icf.empty()) {
-            if (icf.size() < 4 || icf.compare(icf.size() - 4, 4, ".npy") != 0)
-                return Fail("initial condition file '" + icf + "': only .npy is supported by this build "
-                            "(HDF5 ingestion is not built)");
             std::vector<double> d;
-            std::vector<size_t> sh;
-            ZOOMY_TRY(ReadNpy(icf, d, sh));
-            if (sh.size() != 2 || (int)sh[0] != nc) return Fail(icf + ": expected shape (ncells, k)");
-            std::vector<int> fields = settings.io.initial_condition_mask;
-            if (fields.empty())
+            size_t cols = 0;
+            auto ends = [&](const char* suf) {
+                const size_t n = std::strlen(suf);
+                return icf.size() >= n && icf.compare(icf.size() - n, n, suf) == 0;
+            };
+            if (ends(".npy")) {
+                std::vector<size_t> sh;
+                ZOOMY_TRY(ReadNpy(icf, d, sh));
+                if (sh.size() != 2 || (int)sh[0] != nc) return Fail(icf + ": expected shape (ncells, k)");
+                cols = sh[1];
+            } else {
+                ZOOMY_TRY(ReadH5State(icf, d, nc, nq));
+                cols = (size_t)nq;
+            }
+            if (verbose) std::cout << "[INFO] Loading initial condition from " << icf << "..." << std::endl;
+            std::vector<int> fields;
+            for (int k : settings.io.initial_condition_mask)
+                if (k >= 0 && k < nq) fields.push_back(k);
+            const bool all = settings.io.initial_condition_mask.empty();
+            if (all)
                 for (int i = 0; i < nq; ++i) fields.push_back(i);
-            if (sh[1] < fields.size()) return Fail(icf + ": fewer columns than fields");
+            if (cols != (size_t)nq && cols < fields.size()) return Fail(icf + ": fewer columns than fields");
             for (int c = 0; c < nc; ++c) {
-                const int src = m->cell_perm[c];
+                const size_t src = (size_t)m->cell_perm[c] * cols;
                 for (size_t k = 0; k < fields.size(); ++k)
-                    if (fields[k] >= 0 && fields[k] < nq) X[(size_t)c * nq + fields[k]] = d[(size_t)src * sh[1] + k];
+                    X[(size_t)c * nq + fields[k]] = d[src + (cols == (size_t)nq ? (size_t)fields[k] : k)];
             }
             return 0;
         }
@@ -434,6 +469,15 @@ public:
             if (zdm_model_initial_condition(m->cell_centroid + 3 * (size_t)c, parameters.data(), &X[(size_t)c * nq]))
                 return Fail(zdm_last_error());
         return 0;
+    }
+
+    // the host state X (solver cell order) in the mesh file's cell order, AoS n_q per cell
+    void HostStateInInputOrder(std::vector<double>& out) const {
+        const zdm_mesh* m = mesh.get();
+        const int nc = m->ncells, nq = info.n_q;
+        out.assign(X.size(), 0.0);
+        for (int c = 0; c < nc; ++c)
+            for (int i = 0; i < nq; ++i) out[(size_t)m->cell_perm[c] * nq + i] = X[(size_t)c * nq + i];
     }
 
     // VirtualSolver::ComputeTimeStep (VirtualSolver.hpp:285-357) on the resident state

@@ -54,6 +54,30 @@ def test_cli_muscl_equals_python_solver(tmp_path, positivity, ti):
     assert np.array_equal(Xc, input_order_state(mesh, Xp))
 
 
+@pytest.mark.parametrize("mask", [None, [0, 1]])
+def test_cli_hdf5_initial_condition_equals_python_solver(tmp_path, mask):
+    """io.initial_condition_file = an HDF5 "state" file (tests/golden/bump_2d_ic.h5, written by
+    tests/cpp/h5_ic_write.cpp) with and without initial_condition_mask: 20 steps of the CLI and
+    of the Python solver started from zdm.initial_state on the same settings, bitwise equal."""
+    io = {"mesh_path": MESH, "initial_condition_file": os.path.join(ROOT, "tests", "golden", "bump_2d_ic.h5")}
+    if mask is not None:
+        io["initial_condition_mask"] = mask
+    settings = {"name": "cli", "io": io,
+                "solver": {"t_end": 1e9, "cfl": 0.45, "reconstruction_order": 2, "ssp_stages": 2, "limiter": "tvd"}}
+    Xc, log = run_cli("sme1", settings, tmp_path, "-max_steps", "20", "-no_output")
+    assert "Loading initial condition" in log
+    mesh = zdm.Mesh.read_msh(MESH)
+    X0 = zdm.initial_state(settings, mesh, "sme1")
+    s = zdm.Solver("sme1", mesh, settings=zdm.Settings(settings))
+    s.upload(X0)
+    s.initialize()
+    s.step(20)
+    Xp, _ = s.download()
+    s.close()
+    assert np.all(np.isfinite(Xp))
+    assert np.array_equal(Xc, input_order_state(mesh, Xp))
+
+
 def test_cli_chorin_equals_python_solver(tmp_path):
     mesh = zdm.Mesh.read_msh(MESH)
     Xs = cases.c3_state(mesh)

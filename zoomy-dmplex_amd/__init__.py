@@ -95,6 +95,10 @@ def mesh_lib():
         lib.zdm_mesh_last_error.restype = ctypes.c_char_p
         lib.zdm_splitmix64_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
         lib.zdm_splitmix64_next.restype = ctypes.c_uint64
+        lib.zdm_h5_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)]
+        lib.zdm_h5_write.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_uint64)]
         _mesh_lib = lib
     return _mesh_lib
 
@@ -417,6 +421,85 @@ class ModelInfo:
 
 def _dp(a):
     return a.ctypes.data_as(_c_dbl_p)
+
+
+# ------------------------------------------------------------------- initial conditions
+
+def read_h5(path, dataset="state"):
+    """A float64 HDF5 dataset as a numpy array of its own shape (libhdf5 opened at run time,
+    see zdm_mesh.h)."""
+    lib = mesh_lib()
+    nd = ctypes.c_int(0)
+    dims = (ctypes.c_uint64 * 4)()
+    if lib.zdm_h5_read(os.fsencode(path), dataset.encode(), None, 0, ctypes.byref(nd), dims):
+        raise ZdmError(lib.zdm_mesh_last_error().decode())
+    shape = tuple(int(dims[i]) for i in range(nd.value))
+    out = np.empty(shape, dtype=np.float64)
+    if lib.zdm_h5_read(os.fsencode(path), dataset.encode(), out.ctypes.data_as(ctypes.c_void_p), out.size,
+                       ctypes.byref(nd), dims):
+        raise ZdmError(lib.zdm_mesh_last_error().decode())
+    return out
+
+
+def write_h5(path, array, dataset="state"):
+    """One float64 dataset (1 to 4 dimensions) into a new HDF5 file."""
+    a = np.ascontiguousarray(array, dtype=np.float64)
+    if not 1 <= a.ndim <= 4:
+        raise ZdmError("write_h5: 1 to 4 dimensions")
+    dims = (ctypes.c_uint64 * 4)(*a.shape)
+    lib = mesh_lib()
+    if lib.zdm_h5_write(os.fsencode(path), dataset.encode(), a.ctypes.data_as(ctypes.c_void_p), a.ndim, dims):
+        raise ZdmError(lib.zdm_mesh_last_error().decode())
+
+
+def initial_state(settings, mesh, model):
+    """VirtualSolver::SetupInitialConditions (VirtualSolver.hpp:234-281): the state in the
+    mesh's (solver) cell order, (ncells, n_q).  With io.initial_condition_file it is read as
+    IOManager::LoadSolution reads it (IOManager.hpp:122-192): HDF5 dataset "state" of ncells *
+    n_q values, cell-major in the mesh file's cell order (flat or (ncells, n_q)), mapped by
+    cell_perm; io.initial_condition_mask selects components (the others stay 0).  A .npy of
+    shape (ncells, k) is read the same way for k = n_q, and for k < n_q its columns are the
+    mask's components in order.  Without a file, Model::initial_condition at the centroids."""
+    info = model if isinstance(model, ModelInfo) else ModelInfo(model)
+    st = settings if isinstance(settings, Settings) else Settings(settings)
+    nc, nq = mesh.ncells, info.n_q
+    X = np.zeros((nc, nq))
+    path = st.io["initial_condition_file"]
+    if path:
+        if path.endswith(".npy"):
+            d = np.load(path, allow_pickle=False)
+            if d.ndim != 2 or d.shape[0] != nc:
+                raise ZdmError("%s: expected shape (ncells, k)" % path)
+        else:
+            if not os.path.exists(path):
+                raise ZdmError("initial condition file %s cannot be opened" % path)
+            d = read_h5(path, "state")
+            if d.size != nc * nq or d.ndim > 2 or (d.ndim == 2 and d.shape[1] != nq):
+                raise ZdmError('%s: dataset "state" has %d values; the mesh and model need %d x %d'
+                               % (path, d.size, nc, nq))
+            d = d.reshape(nc, nq)
+        mask = st.io["initial_condition_mask"]
+        fields = [k for k in mask if 0 <= k < nq] if mask else list(range(nq))
+        src = d[mesh.cell_perm]
+        if d.shape[1] == nq:
+            X[:, fields] = src[:, fields]
+        else:
+            if d.shape[1] < len(fields):
+                raise ZdmError("%s: fewer columns than fields" % path)
+            X[:, fields] = src[:, :len(fields)]
+        return X
+    lib = model_lib(info.name)
+    p = np.array(info.default_params, dtype=np.float64)
+    for k, v in st.parameters.items():
+        if k in info.parameter_names:
+            p[info.parameter_names.index(k)] = v
+    cen = np.ascontiguousarray(mesh.cell_centroid, dtype=np.float64)
+    row = np.zeros(nq)
+    for c in range(nc):
+        if lib.zdm_model_initial_condition(_dp(cen[c]), _dp(p), _dp(row)):
+            raise ZdmError(lib.zdm_last_error().decode())
+        X[c] = row
+    return X
 
 
 class Solver:

@@ -1,6 +1,6 @@
 """Build the in-tree shared libraries of the package.
 
-  libzdm_mesh.so          host mesh service (g++)
+  libzdm_mesh.so          host mesh service and HDF5 initial-condition files (g++)
   libzdm_<model>.so       HIP time step for gfx950, one per model directory under models/
 
 Models are compiled exactly as the reference compiles its solver on the generated headers:
@@ -35,7 +35,7 @@ def _run(cmd):
 def build_mesh():
     out = os.path.join(HERE, "libzdm_mesh.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-I" + INCLUDE,
-          os.path.join(HERE, "csrc", "mesh.cpp"), "-o", out])
+          os.path.join(HERE, "csrc", "mesh.cpp"), os.path.join(HERE, "csrc", "h5io.cpp"), "-ldl", "-o", out])
     return out
 
 

@@ -405,6 +405,10 @@ int build(int dim, int nverts, const double* xyz, int ncells, int nvpc, const in
 
 }  // namespace
 
+// errors of the other host services of this library (h5io.cpp)
+void zdm_mesh_set_error(const std::string& msg) { g_err = msg; }
+
+
 extern "C" {
 
 uint64_t zdm_splitmix64_next(uint64_t* s) {

@@ -53,7 +53,7 @@ static void print_settings(const zoomy::Settings& s) {
 int main(int argc, char** argv) {
     std::string spath = "settings.json", out;
     int max_steps = 1 << 30;
-    bool print = false, snapshots = true;
+    bool print = false, snapshots = true, ic_only = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "-settings" && i + 1 < argc) spath = argv[i + 1];
@@ -61,6 +61,7 @@ int main(int argc, char** argv) {
         if (a == "-max_steps" && i + 1 < argc) max_steps = std::stoi(argv[i + 1]);
         if (a == "-print-settings") print = true;
         if (a == "-no_output") snapshots = false;
+        if (a == "-ic_only") ic_only = true;   // load the mesh and initial condition, write it (-o), stop
     }
     zoomy::Settings st;
     try {
@@ -80,6 +81,20 @@ int main(int argc, char** argv) {
         solver = std::make_unique<zoomy::MUSCLSolver>();
     solver->max_steps = max_steps;
     solver->write_snapshots = snapshots;
+    if (ic_only) {
+        // the host setup only (settings, mesh, initial condition): no device work
+        std::vector<double> Xo;
+        if (solver->Initialize(argc, argv)) {
+            std::cerr << "Error: " << zoomy::LastError() << std::endl;
+            return 2;
+        }
+        solver->HostStateInInputOrder(Xo);
+        if (!out.empty() && zoomy::WriteNpy(out, Xo, (size_t)solver->mesh.ncells(), (size_t)solver->info.n_q)) {
+            std::cerr << "Error: " << zoomy::LastError() << std::endl;
+            return 3;
+        }
+        return 0;
+    }
     if (solver->Run(argc, argv)) {
         std::cerr << "Error: " << zoomy::LastError() << std::endl;
         return 2;
