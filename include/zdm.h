@@ -287,13 +287,17 @@ int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape);
  * most `steps` steps (one segment for checkpoint_every >= 2) is re-run once in the reverse
  * sweep with its stage states, post-stage states and (level 2) face states recorded in place,
  * and its adjoints then run only transposed kernels, instead of each reversed step re-running
- * its own step; the forward run records the last stretch itself.  zdm_adjoint_setup picks
- * one: ceil(nsteps / c) steps on c binomial slots (one segment for checkpoint_every >= 2), at
- * level 2 if it fits in half of the free device memory, else level 1, else none.  steps >= 0
- * replaces it (0: none), re-planning the schedule; steps < 0 only queries.  steps_out /
- * level_out (may be NULL) receive its length and level (0: none).  The gradient is bitwise
- * the same with or without it.  Not for the Chorin split.  No counterpart in the reference. */
-int zdm_adjoint_stretch_tape(zdm_solver* s, int steps, int* steps_out, int* level_out);
+ * its own step; the forward run records the last stretch itself.  With two buffers the
+ * re-runs run on a second stream of the solver: the next stretch is recorded while the
+ * current one is reversed (level 2 only).  zdm_adjoint_setup picks ceil(nsteps / c) steps on
+ * c binomial slots (one segment for checkpoint_every >= 2): two level-2 buffers if they fit in
+ * half of the free device memory, else one at level 2, else level 1, else none.  steps >= 0
+ * replaces it (0: none; buffers 0 automatic, 1 or 2), re-planning the schedule; steps < 0 only
+ * queries.  steps_out / level_out / buffers_out (may be NULL) receive its length, level and
+ * buffers (0: none).  The gradient is bitwise the same with or without it.  Not for the
+ * Chorin split.  No counterpart in the reference. */
+int zdm_adjoint_stretch_tape(zdm_solver* s, int steps, int buffers, int* steps_out, int* level_out,
+                             int* buffers_out);
 /* Forward run (X0, A0 in the zdm_state_upload layout, A0 may be NULL).  dts = NULL takes
  * dt = CFL each step and records it; otherwise the given sequence is used (a frozen-dt
  * re-run, e.g. for finite differences).  Returns the loss. */

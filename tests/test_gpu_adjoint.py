@@ -85,15 +85,15 @@ def make_layout(order, steps, ck, stretch, exact):
     return s, X, A
 
 
-# (checkpoint_every, stretch tape: None automatic / 0 none / L, expected layout, expected
-# stretch tape (steps, level)); 8 steps
-LAYOUTS_8 = [(-2, None, (1, 2), (0, 0)), (-1, None, (1, 1), (0, 0)), (0, None, None, None),
-             (1, None, (1, 0), (0, 0)), (3, None, (3, 0), (3, 2)), (3, 0, (3, 0), (0, 0)),
-             (8, None, (8, 0), (8, 2)), (8, 0, (8, 0), (0, 0)),
-             (-3, None, (-3, 0), (8, 2)), (-3, 0, (-3, 0), (0, 0)), (-3, 3, (-3, 0), (3, 2)),
-             (-4, None, (-4, 0), (4, 2)), (-4, 0, (-4, 0), (0, 0)), (-4, 1, (-4, 0), (1, 2)),
-             (-5, None, (-5, 0), (3, 2)), (-5, 0, (-5, 0), (0, 0)), (-5, 2, (-5, 0), (2, 2)),
-             (-7, None, (-7, 0), (2, 2)), (-7, 0, (-7, 0), (0, 0))]
+# (checkpoint_every, stretch tape: None automatic / 0 none / L / (L, buffers), expected layout,
+# expected stretch tape (steps, level, buffers)); 8 steps
+LAYOUTS_8 = [(-2, None, (1, 2), (0, 0, 0)), (-1, None, (1, 1), (0, 0, 0)), (0, None, None, None),
+             (1, None, (1, 0), (0, 0, 0)), (3, None, (3, 0), (3, 2, 2)), (3, 0, (3, 0), (0, 0, 0)),
+             (3, (3, 1), (3, 0), (3, 2, 1)), (8, None, (8, 0), (8, 2, 2)), (8, 0, (8, 0), (0, 0, 0)),
+             (-3, None, (-3, 0), (8, 2, 2)), (-3, 0, (-3, 0), (0, 0, 0)), (-3, 3, (-3, 0), (3, 2, 2)),
+             (-4, None, (-4, 0), (4, 2, 2)), (-4, 0, (-4, 0), (0, 0, 0)), (-4, (1, 1), (-4, 0), (1, 2, 1)),
+             (-4, (1, 2), (-4, 0), (1, 2, 2)), (-5, None, (-5, 0), (3, 2, 2)), (-5, 0, (-5, 0), (0, 0, 0)),
+             (-5, (2, 1), (-5, 0), (2, 2, 1)), (-7, None, (-7, 0), (2, 2, 2)), (-7, 0, (-7, 0), (0, 0, 0))]
 
 
 @pytest.mark.parametrize("exact", [True, False])
@@ -124,10 +124,12 @@ def test_trajectory_layout_does_not_change_the_gradient(order, exact):
 def test_binomial_schedule_on_a_longer_trajectory(exact):
     """21 steps on 3 and 4 slots (split by the dynamic programme, nested several levels deep),
     with and without stretch tapes (2 and 3 steps leave the slots too few to cover the
-    trajectory, so stretches are reversed in chunks from the scratch state), and checkpoints
-    every 4 with a stretch tape: bitwise the gradient of the tape."""
+    trajectory, so stretches are reversed in chunks from the scratch state), with one tape
+    buffer and with two (re-runs on the second stream), and checkpoints every 4 with a stretch
+    tape: bitwise the gradient of the tape."""
     grads = []
-    for ck, stretch in ((-2, None), (-5, 0), (-5, None), (-5, 2), (-6, 0), (-6, None), (-6, 3), (4, None)):
+    for ck, stretch in ((-2, None), (-5, 0), (-5, None), (-5, 2), (-5, (2, 1)), (-6, 0), (-6, None), (-6, 3),
+                        (-6, (3, 2)), (4, None), (4, (4, 1))):
         print("case exact", exact, "checkpoint_every", ck, "stretch", stretch, flush=True)
         s, X, A = make_layout(2, 21, ck, stretch, exact)
         s.adjoint_forward(X, A)

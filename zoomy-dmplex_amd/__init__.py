@@ -375,8 +375,8 @@ def model_lib(model):
     lib.zdm_adjoint_dts.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_residuals.argtypes = [V, _c_dbl_p]
     lib.zdm_adjoint_layout.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
-    lib.zdm_adjoint_stretch_tape.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
-                                             ctypes.POINTER(ctypes.c_int)]
+    lib.zdm_adjoint_stretch_tape.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.zdm_adjoint_forward_device.argtypes = [V, V, V, _c_dbl_p, ctypes.POINTER(ctypes.c_double)]
     lib.zdm_adjoint_gradient_device.argtypes = [V, V, V]
     lib.zdm_solver_stream.argtypes = [V, ctypes.POINTER(ctypes.c_void_p)]
@@ -676,7 +676,8 @@ class Solver:
         checkpoint every k steps, -2-c (c >= 1) the binomial schedule on c state slots, 0 the
         richest layout that fits in device memory.
         stretch_tape (checkpoint_every >= 2 or <= -3): None the automatic choice, 0 none, L a
-        tape of L steps (one segment for checkpoint_every >= 2), see adjoint_stretch_tape."""
+        tape of L steps (one segment for checkpoint_every >= 2), or (L, buffers); see
+        adjoint_stretch_tape."""
         g = np.ascontiguousarray(gauges, dtype=np.int32)
         nobs = nsteps // obs_every if len(g) and obs_every else 0
         o = np.zeros((nobs, len(g))) if obs is None else np.ascontiguousarray(obs, dtype=np.float64)
@@ -687,15 +688,19 @@ class Solver:
         self._check(fn(self._h, int(nsteps), int(checkpoint_every), int(obs_every), len(g),
                        g.ctypes.data_as(_c_int_p), _dp(o) if o.size else None, int(bool(exact_limiter))))
         if stretch_tape is not None:
-            self.adjoint_stretch_tape(int(stretch_tape))
+            st = stretch_tape if isinstance(stretch_tape, tuple) else (stretch_tape, 0)
+            self.adjoint_stretch_tape(int(st[0]), int(st[1]))
 
-    def adjoint_stretch_tape(self, steps=-1):
+    def adjoint_stretch_tape(self, steps=-1, buffers=0):
         """Stretch tape of a checkpointed layout: each stretch of up to `steps` steps is re-run
-        once with its stages recorded, then reversed with transposed kernels only.  steps >= 0
-        sets it (0: none), -1 only queries; returns (steps, level), (0, 0) for none."""
-        n, lv = ctypes.c_int(0), ctypes.c_int(0)
-        self._check(self.lib.zdm_adjoint_stretch_tape(self._h, int(steps), ctypes.byref(n), ctypes.byref(lv)))
-        return n.value, lv.value
+        once with its stages recorded, then reversed with transposed kernels only; with two
+        buffers the re-runs overlap the reversal on a second stream.  steps >= 0 sets it (0:
+        none; buffers 0 automatic, 1 or 2), -1 only queries; returns (steps, level, buffers),
+        (0, 0, 0) for none."""
+        n, lv, nb = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.zdm_adjoint_stretch_tape(self._h, int(steps), int(buffers), ctypes.byref(n),
+                                                      ctypes.byref(lv), ctypes.byref(nb)))
+        return n.value, lv.value, nb.value
 
     def implicit_setup(self, **kw):
         """Newton / Krylov settings of time_integration "implicit" (NewtonOptions fields)."""

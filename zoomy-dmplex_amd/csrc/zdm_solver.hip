@@ -3255,15 +3255,18 @@ int zdm_adjoint_layout(zdm_solver* s, int* checkpoint_every, int* tape) {
     return 0;
 }
 
-int zdm_adjoint_stretch_tape(zdm_solver* s, int steps, int* steps_out, int* level_out) {
+int zdm_adjoint_stretch_tape(zdm_solver* s, int steps, int buffers, int* steps_out, int* level_out,
+                             int* buffers_out) {
     if (!s || !s->adj) return fail("zdm_adjoint_stretch_tape: call zdm_adjoint_setup first");
     AdjointHost* ad = s->adj;
+    if (buffers < 0 || buffers > 2) return fail("zdm_adjoint_stretch_tape: buffers must be 0, 1 or 2");
     if (steps >= 0) {
         if (ad->tape && !ad->stretch) return fail("zdm_adjoint_stretch_tape: the layout is a full tape");
-        if (stretch_setup(s, ad, steps, false)) return -1;
+        if (stretch_setup(s, ad, steps, false, buffers)) return -1;
     }
     if (steps_out) *steps_out = ad->stretch ? ad->tlen : 0;
     if (level_out) *level_out = ad->stretch ? ad->tape : 0;
+    if (buffers_out) *buffers_out = ad->stretch ? ad->tbufs : 0;
     return 0;
 }
 
