@@ -583,24 +583,53 @@ def c5_default_layout(T):
     return -2 - int(np.ceil(np.sqrt(T)))
 
 
+def _device_arrays(*arrays):
+    """float64 device copies (torch) of host arrays, or None without a GPU torch."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        out = [torch.tensor(a, dtype=torch.float64, device="cuda") for a in arrays]
+        torch.cuda.synchronize()
+        return out
+    except ImportError:
+        return None
+
+
 def c5_measure(s, T, gauges, obs_every, obs, X, A, ck, exact, stretch=None):
-    """Recording forward + reverse sweep, once to warm up and once timed."""
+    """Recording forward + reverse sweep, once to warm up and once timed, with the inputs and
+    the gradient in device memory (zdm_adjoint_forward_device / _gradient_device) when torch
+    has the GPU, else through host arrays."""
     s.adjoint_setup(T, gauges=gauges, obs_every=obs_every, obs=obs, checkpoint_every=ck,
                     exact_limiter=exact, stretch_tape=stretch)
-    s.adjoint_forward(X, A)
-    s.adjoint_gradient()
+    dev = _device_arrays(X, A, np.zeros_like(X), np.zeros_like(A))
+    if dev is not None:
+        Xd, Ad, gXd, gAd = dev
+        fwd = lambda: s.adjoint_forward_device(Xd.data_ptr(), Ad.data_ptr())
+        rev = lambda: s.adjoint_gradient_device(gXd.data_ptr(), gAd.data_ptr())
+    else:
+        fwd = lambda: s.adjoint_forward(X, A)
+        rev = lambda: s.adjoint_gradient()
+    fwd()
+    rev()
     s.synchronize()
     t0 = time.perf_counter()
-    loss = s.adjoint_forward(X, A)
+    loss = fwd()
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
-    gX, gA = s.adjoint_gradient()
+    rev()
+    s.synchronize()
     t_rev = time.perf_counter() - t0
+    if dev is not None:
+        gX, gA = gXd.cpu().numpy(), gAd.cpu().numpy()
+    else:
+        gX, gA = s.adjoint_gradient()
     if not (np.all(np.isfinite(gX)) and np.all(np.isfinite(gA))):
         raise RuntimeError("non-finite gradient (checkpoint_every %d, exact %s)" % (ck, exact))
     lay = s.adjoint_layout()
     st = s.adjoint_stretch_tape()
-    return {"t_fwd": t_fwd, "t_rev": t_rev, "loss": loss, "gX": gX, "gA": gA, "layout": lay, "stretch": st}
+    return {"t_fwd": t_fwd, "t_rev": t_rev, "loss": loss, "gX": gX, "gA": gA, "layout": lay, "stretch": st,
+            "inputs": "device" if dev is not None else "host"}
 
 
 def c5_layout_name(m):
@@ -704,6 +733,7 @@ def run_c5(zdm, cases, args):
             "grad_norm_manning": float(np.linalg.norm(gA[:, 1])),
             "checkpoint_every": head["layout"][0],
             "layout": c5_layout_name(head),
+            "inputs_and_gradient": head["inputs"] + " memory",
             "breakdown": breakdown,
         },
         "roofline": None,

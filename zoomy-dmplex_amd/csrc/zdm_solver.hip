@@ -1521,7 +1521,8 @@ __global__ __launch_bounds__(FBLOCK) void k_finalize(Ctrl* ctrl, const T* __rest
 // result does not depend on the arrival order.
 __global__ __launch_bounds__(FBLOCK) void k_finalize_xcd(Ctrl* ctrl, const T* __restrict__ partials, int np,
                                                          T* xmin, unsigned* arrived, int what, int order,
-                                                         int s, T cfl, T min_dt, T t_end, T dt_max) {
+                                                         int s, T cfl, T min_dt, T t_end, T dt_max,
+                                                         T* dt_log) {
     if ((what == 2 || what == 3) && !ctrl->active) return;
     const int x = blockIdx.x, q = np / NXCD, r = np % NXCD;
     const int b0 = x * q + (x < r ? x : r), b1 = b0 + q + (x < r ? 1 : 0);
@@ -1538,6 +1539,7 @@ __global__ __launch_bounds__(FBLOCK) void k_finalize_xcd(Ctrl* ctrl, const T* __
     }
     *arrived = 0u;
     finish_step(ctrl, vmin, what, order, s, cfl, min_dt, t_end, dt_max);
+    if (dt_log) *dt_log = ctrl->dt;   // the dt of the next step (optional record)
 }
 
 __global__ void k_aos_to_soa(const T* __restrict__ aos, T* __restrict__ soa, int n, int nc, int ld) {
@@ -1651,6 +1653,7 @@ struct zdm_solver {
     T* Im;                            // IMEX: stage right-hand side, stage state, running sums P3, F
     double cfl_in;                    // solver.cfl as given (opt.cfl: capped for zhang_shu)
     Ctrl *ctrl, *ctrl_rhs;
+    T* dt_log = nullptr;              // single run: finalize also stores the next dt here (adjoint forward)
     bool timing;                      // record kernel events on the step being enqueued
     int timing_stride;                // zdm_enable_timing: events on every timing_stride-th step
     long long timing_step;
@@ -1955,7 +1958,7 @@ void finalize(zdm_solver* s, int what) {
         k_finalize_xcd<<<NXCD, FBLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, xmin,
                                                        reinterpret_cast<unsigned*>(xmin + NXCD), what,
                                                        s->opt.order, s->opt.ssp_stages, s->opt.cfl,
-                                                       s->opt.min_dt, s->opt.t_end, (T)M::dt_max);
+                                                       s->opt.min_dt, s->opt.t_end, (T)M::dt_max, s->dt_log);
         return;
     }
     k_finalize<<<s->nctrl, FBLOCK, 0, s->stream>>>(s->ctrl, s->partials, s->nblocks, s->dm.blk_first,

@@ -247,6 +247,7 @@ struct SmeNumerics2D {
         T h;                    // h*
         T m[2 * NM + NTR];      // reconstructed momenta (x moments, y moments) and tracers
         T u[2 * NM];            // velocities            (x moments then y moments)
+        T r, hi;                // h* / max(1e-14, h) and hinv_kp(h*), kept for the derivative
     };
 
     PORTABLE_FN static inline void reconstruct(const T* Q, T bstar, HR& s) {
@@ -258,6 +259,8 @@ struct SmeNumerics2D {
             s.u[k] = s.m[k] * hi;
         }
         for (int k = 0; k < NTR; ++k) s.m[2 * NM + k] = Q[IT + k] * r;
+        s.r = r;
+        s.hi = hi;
     }
 
     // normal flux F(q*) . n of a reconstructed state (b slot is zero)
@@ -444,9 +447,9 @@ struct SmeNumerics2D {
     // when a < b, fabs(a) has slope -1 when a < 0 (the choices forward-mode duals make through
     // the functions above), so the result is the exact transpose of their Jacobian.
 
-    // d hinv_kp(h, eps) / dh times bres
-    PORTABLE_FN static inline T hinv_kp_vjp(T h, T eps, T bres) {
-        if (h >= eps) return -(bres / (h * h));   // d(1/h)/dh
+    // d hinv_kp(h, eps) / dh times bres; hi = hinv_kp(h, eps)
+    PORTABLE_FN static inline T hinv_kp_vjp(T h, T eps, T bres, T hi) {
+        if (h >= eps) return -((bres * hi) * hi);   // d(1/h)/dh = -1/h^2
         const T h2 = h * h, e2 = eps * eps;
         const T den = std::sqrt(h2 * h2 + e2 * e2);
         const T c = (T)1.4142135623730951;
@@ -524,14 +527,13 @@ struct SmeNumerics2D {
     PORTABLE_FN static inline T reconstruct_vjp(const T* Q, T bstar, const HR& s, T bh, T* bm, const T* bu,
                                                 T* g) {
         const T den = std::max((T)1.0e-14, Q[1]);
-        const T r = s.h / den;
-        const T hi = M::hinv_kp(s.h, eps_flux);
+        const T r = s.r, hi = s.hi;
         T bhi = 0;
         for (int k = 0; k < 2 * NM; ++k) {
             bm[k] += hi * bu[k];
             bhi += s.m[k] * bu[k];
         }
-        bh += hinv_kp_vjp(s.h, eps_flux, bhi);
+        bh += hinv_kp_vjp(s.h, eps_flux, bhi, hi);
         T br = 0;
         for (int k = 0; k < 2 * NM; ++k) {
             g[IU + k] += r * bm[k];
@@ -541,8 +543,9 @@ struct SmeNumerics2D {
             g[IT + k] += r * bm[2 * NM + k];
             br += Q[IT + k] * bm[2 * NM + k];
         }
-        bh += br / den;
-        if ((T)1.0e-14 < Q[1]) g[1] -= r * br / den;
+        const T brd = br / den;   // d r / d h* = 1 / den, d r / d h = -r / den
+        bh += brd;
+        if ((T)1.0e-14 < Q[1]) g[1] -= r * brd;
         if ((T)0 < Q[0] + Q[1] - bstar) {
             g[0] += bh;
             g[1] += bh;
@@ -626,7 +629,7 @@ struct SmeNumerics2D {
                     bdq[IU + k] += gt[q] * (hiq[q] * w);
                     bhi += (sL.m[k] + gt[q] * dq[IU + k]) * w;
                 }
-                const T bh = hinv_kp_vjp(hq[q], eps_flux, bhi);
+                const T bh = hinv_kp_vjp(hq[q], eps_flux, bhi, hiq[q]);
                 bhL += bh;
                 bdq[1] += gt[q] * bh;
             }
