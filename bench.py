@@ -82,11 +82,33 @@ sys.path.insert(0, sys.argv[1])
 from conftest import oracle, zdm, cases
 import numpy as np
 workload, arg, steps = sys.argv[2], int(sys.argv[3]), int(sys.argv[4])
+extra = json.loads(sys.argv[5]) if len(sys.argv) > 5 else {}
+if workload == "c3":
+    orc = oracle("vam")
+    mesh = zdm.Mesh.tri_grid(arg, arg, jitter=0.15, seed=3)
+    X = cases.c3_state(mesh)
+    A = np.zeros((mesh.ncells, orc.n_aux))
+    fb = zdm.boundary_face_bc(mesh, ["default"], 2)
+    runs, its = [], []
+    for n in (steps, 2 * steps):
+        Xc, Ac, Pc = X.copy(), A.copy(), np.zeros((mesh.ncells, 2))
+        t0 = time.perf_counter()
+        _, _, it, reason = orc.chorin_run(mesh, fb, orc.default_params, Xc, Ac, Pc, order=1, max_steps=n, **extra)
+        runs.append(time.perf_counter() - t0)
+        its.append(it)
+    print(json.dumps({"cells": mesh.ncells, "per_step": (runs[1] - runs[0]) / steps,
+                      "its_per_step": (its[1] - its[0]) / steps}))
+    sys.exit(0)
 if workload == "c2":
     model = "sme2"
     mesh = zdm.Mesh.tri_grid(arg, arg)
     info = zdm.ModelInfo(model)
     X = cases.c2_state(mesh, info.n_q, seed=2)
+elif workload == "c1":
+    model = "sme0"
+    mesh = zdm.Mesh.tri_grid(arg, arg)
+    info = zdm.ModelInfo(model)
+    X = cases.c1_state(mesh, 4, seed=1)
 else:
     model = "sme1"
     n, mesh_seed, state = cases.ensemble_member(arg)
@@ -124,11 +146,12 @@ def host_cpu():
     return {"os_cpu_count": os.cpu_count(), "cpu_model": model}
 
 
-def _oracle_procs(args_list, steps):
+def _oracle_procs(args_list, steps, extra=None):
     """One oracle process per entry of args_list (workload, arg), all at once."""
     import subprocess
     env = dict(os.environ, ZDM_NO_TORCH="1", OMP_NUM_THREADS="1")
-    ps = [subprocess.Popen([sys.executable, "-c", _CPU_CHILD, os.path.join(ROOT, "tests"), w, str(a), str(steps)],
+    tail = [json.dumps(extra)] if extra else []
+    ps = [subprocess.Popen([sys.executable, "-c", _CPU_CHILD, os.path.join(ROOT, "tests"), w, str(a), str(steps)] + tail,
                            stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, text=True)
           for w, a in args_list]
     res = []
@@ -140,13 +163,13 @@ def _oracle_procs(args_list, steps):
     return res
 
 
-def cpu_baseline(zdm, cases, grid=250, steps=20, procs=None):
+def cpu_baseline(zdm, cases, grid=250, steps=20, procs=None, workload="c2"):
     """The oracle (C++ restatement of the reference step, one thread each) run as `procs`
     concurrent processes on the host's cores (at most the 16-core share of one GPU), each on
     the same bounded sample; the aggregate throughput is reported (slope of T vs 2T steps
     removes start-up)."""
     procs = procs or max(1, min(CPU_SHARE_PER_GPU, os.cpu_count() or 1))
-    res = _oracle_procs([("c2", grid)] * procs, steps)
+    res = _oracle_procs([(workload, grid)] * procs, steps)
     cells = res[0]["cells"]
     per_step = sum(r["per_step"] for r in res) / len(res)
     return dict({
@@ -223,34 +246,64 @@ def c5_cpu_baseline(zdm, cases, grid=120, steps=10):
     }, **host_cpu())
 
 
-def c3_cpu_baseline(zdm, cases, grid=60, steps=1, krylov=None):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from conftest import oracle  # CPU checker, timed here as the reported baseline
-    orc = oracle("vam")
-    mesh = zdm.Mesh.tri_grid(grid, grid, jitter=0.15, seed=3)
-    X = cases.c3_state(mesh)
-    A = np.zeros((mesh.ncells, orc.n_aux))
-    fb = zdm.boundary_face_bc(mesh, ["default"], 2)
-    runs, its = [], []
-    for n in (steps, 2 * steps):
-        Xc, Ac, Pc = X.copy(), A.copy(), np.zeros((mesh.ncells, 2))
-        t0 = time.perf_counter()
-        _, _, it, reason = orc.chorin_run(mesh, fb, orc.default_params, Xc, Ac, Pc, order=1,
-                                          max_steps=n, **krylov)
-        runs.append(time.perf_counter() - t0)
-        its.append(it)
-    per_step = (runs[1] - runs[0]) / steps
-    per_it = (runs[1] - runs[0]) / max(its[1] - its[0], 1)
-    return {
-        "value": mesh.ncells / per_step / 1e6,
+def c3_cpu_baseline(zdm, cases, grid=60, steps=1, krylov=None, procs=None):
+    """The oracle's ChorinVAMSolver (sequential GMRES, one thread each) as `procs` concurrent
+    processes on the same bounded sample, like cpu_baseline; aggregate Mcell-steps/s."""
+    procs = procs or max(1, min(CPU_SHARE_PER_GPU, os.cpu_count() or 1))
+    res = _oracle_procs([("c3", grid)] * procs, steps, extra=krylov)
+    cells = res[0]["cells"]
+    per_step = sum(r["per_step"] for r in res) / len(res)
+    its = res[0]["its_per_step"]
+    return dict({
+        "value": procs * cells / per_step / 1e6,
         "unit": "Mcell-steps/s",
-        "cores": 1,
+        "cores": procs,
         "kind": "port",
-        "mcell_its_per_s": round(mesh.ncells / per_it / 1e6, 3),
-        "sample": "oracle (C++ restatement of ChorinVAMSolver, sequential GMRES, 1 thread) on a "
-                  "%dx%d jittered grid (%d cells), slope of %d vs %d steps, %.1f GMRES its/step"
-                  % (grid, grid, mesh.ncells, steps, 2 * steps, (its[1] - its[0]) / steps),
+        "mcell_its_per_s": round(procs * cells * its / per_step / 1e6, 3),
+        "sample": "oracle (C++ restatement of ChorinVAMSolver, sequential GMRES, one thread per process) as "
+                  "%d concurrent processes (the 16-core share of one GPU), each on a %dx%d jittered grid "
+                  "(%d cells), slope of %d vs %d steps, %.1f GMRES its/step"
+                  % (procs, grid, grid, cells, steps, 2 * steps, its),
+    }, **host_cpu())
+
+
+def run_c1(zdm, cases, args):
+    """C1: SME level 0 dam break over a bump, 60 x 60 grid (7 200 cells), SSP-RK2, CFL 0.45,
+    200 steps, replayed as a captured step graph (a grid this small is launch-bound)."""
+    n = args.grid if args.grid != 1000 else 60
+    mesh = zdm.Mesh.tri_grid(n, n, renumber=args.order)
+    X = cases.c1_state(mesh, 4, seed=1)
+    s = zdm.Solver("sme0", mesh, order=2, ssp_stages=2, cfl=0.45, t_end=1e9)
+    s.upload(X, None)
+    s.initialize()
+    s.step_graph(args.warmup)
+    s.synchronize()
+    s.upload(X, None)
+    s.initialize()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step_graph(args.steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    Xg, _ = s.download()
+    s.close()
+    if not np.all(np.isfinite(Xg)):
+        raise RuntimeError("non-finite C1 state")
+    out = {
+        "metric": "Mcell-steps/s (SME L0, FP64)",
+        "value": round(mesh.ncells * args.steps / dt / 1e6, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "SME level 0 dam break over a Gaussian bump, %dx%d right-triangle grid (%d cells), "
+                               "MUSCL + TVD limiter, SSP-RK2, CFL 0.45, graph-replayed steps" % (n, n, mesh.ncells),
+                   "cells": mesh.ncells},
+        "roofline": None,
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(zdm, cases, grid=n, steps=100, workload="c1")
+    print(json.dumps(out))
 
 
 def run_c3(zdm, cases, args):
@@ -818,8 +871,9 @@ def run_c5_fd(zdm, cases, args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5fd", "c3adj"],
-                    help="c2: SME level 2 transport + source (headline); c3: VAM Chorin split; "
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "c5fd", "c3adj"],
+                    help="c1: SME level 0 on 60x60 (the CPU case); c2: SME level 2 transport + source (headline); "
+                         "c3: VAM Chorin split; "
                          "c4: batched vs serial ensemble; c5: adjoint calibration gradient; "
                          "c5fd: its central-difference check; c3adj: adjoint through the Chorin split")
     ap.add_argument("--members", type=int, default=256, help="c4 ensemble size")
@@ -863,9 +917,11 @@ def main():
         sys.exit(2)
 
     if args.steps is None:
-        args.steps = {"c2": 200, "c3": 10, "c4": 300, "c5": 500, "c5fd": 0, "c3adj": 10}[args.workload]
+        args.steps = {"c1": 200, "c2": 200, "c3": 10, "c4": 300, "c5": 500, "c5fd": 0, "c3adj": 10}[args.workload]
     zdm = load_package()
     import zoomy_dmplex_amd.cases as cases
+    if args.workload == "c1":
+        return run_c1(zdm, cases, args)
     if args.workload == "c5":
         return run_c5(zdm, cases, args)
     if args.workload == "c5fd":
