@@ -54,6 +54,14 @@ def flux_launch_bytes_per_cell(n_q, faces_per_cell, stages=2):
     return flux_pass_bytes_per_cell(n_q, faces_per_cell) + 8.0 * n_q / stages
 
 
+def final_post_bytes_per_cell(n_q, faces_per_cell):
+    """Algorithmic HBM bytes per cell of the final stage fused with the post-step (k_flux_post):
+    a flux pass, Q^n for the SSP combination (8 n), the source's read and write of the state
+    (16 n) and the CFL pass (8 n + 8), the step's terms of those passes; the kernel itself moves
+    less, since the new state stays in registers between them."""
+    return flux_pass_bytes_per_cell(n_q, faces_per_cell) + 32 * n_q + 8
+
+
 def step_bytes_per_cell(n_q, faces_per_cell, stages=2):
     """Algorithmic HBM bytes of one full step per cell:
     s (88 n + 96 + 56 F) + 8 n (Q^n in the last stage) + 16 n (source) + 8 n + 8 (CFL)."""
@@ -966,6 +974,7 @@ def main():
     flux_ms, flux_n = s.kernel_time(1)
     prep_ms, prep_n = s.kernel_time(0)
     post_ms, post_n = s.kernel_time(2)
+    fpost_ms, fpost_n = s.kernel_time(3)
     s.enable_timing(False)
     t_sim, dt, nsteps = s.time()
     Xg, _ = s.download()
@@ -975,8 +984,24 @@ def main():
     ncells = mesh.ncells
     F = mesh.nfaces / ncells
     value = ncells * args.steps / elapsed / 1e6
-    flux_bytes = flux_launch_bytes_per_cell(info.n_q, F) * ncells
-    achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
+    if fpost_n:
+        # the step's two flux passes: k_flux (stage) and k_flux_post (final stage + post-step)
+        nl = flux_n + fpost_n
+        bpc = (flux_n * flux_pass_bytes_per_cell(info.n_q, F) +
+               fpost_n * final_post_bytes_per_cell(info.n_q, F)) / nl
+        launch_ms = (flux_n * flux_ms + fpost_n * fpost_ms) / nl
+        rkernel = "k_flux (stage: flux + fluctuations gather + SSP stage update) and k_flux_post (final " \
+                  "stage fused with the post-step: constraints, Newton source, constraints, CFL)"
+        traffic = pmc_traffic("k_flux+k_flux_post", ncells)
+        tsrc = "rocprofv3 FETCH_SIZE + WRITE_SIZE, mean of the k_flux and k_flux_post launches, " \
+               "copy-kernel calibrated (profiles/k_flux_c2_2M_pmc.md)"
+    else:
+        nl, bpc, launch_ms = flux_n, flux_launch_bytes_per_cell(info.n_q, F), flux_ms
+        rkernel = "k_flux (flux + fluctuations gather + SSP stage update)"
+        traffic = pmc_traffic("k_flux", ncells)
+        tsrc = "rocprofv3 FETCH_SIZE + WRITE_SIZE, mean of the stage and final launches, copy-kernel " \
+               "calibrated (profiles/k_flux_c2_2M_pmc.md)"
+    achieved = bpc * ncells / (launch_ms * 1e-3) / 1e9
     out = {
         "metric": "Mcell-steps/s (SME L2, FP64)",
         "value": round(value, 3),
@@ -999,24 +1024,24 @@ def main():
             "faces_per_cell": round(F, 4),
             "step_bytes_per_cell": round(step_bytes_per_cell(info.n_q, F), 1),
             "kernel_ms": {"prep": round(prep_ms, 4), "flux": round(flux_ms, 4),
-                          "post": round(post_ms, 4)},
-            "kernel_ms_sample": "HIP events on every %d-th timed step (%d flux launches)" % (TIMING_STRIDE, flux_n),
+                          "post": round(post_ms, 4), "flux_post": round(fpost_ms, 4)},
+            "kernel_ms_sample": "HIP events on every %d-th timed step (%d flux, %d flux_post launches)" % (
+                TIMING_STRIDE, flux_n, fpost_n),
             "launch": {"block_size": shape[0], "cells_per_thread": shape[1],
                        "autotune_ms_per_step": {"%dx%d" % k: round(v, 4) for k, v in tune.items()}},
             "sim_time": t_sim,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_flux (flux + fluctuations gather + SSP stage update)",
+            "kernel": rkernel,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": pmc_traffic("k_flux", ncells),
-            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, mean of the stage and final "
-                              "launches, copy-kernel calibrated (profiles/k_flux_c2_2M_pmc.md)",
-            "bytes_per_cell": round(flux_launch_bytes_per_cell(info.n_q, F), 2),
-            "launches": flux_n,
+            "traffic": traffic,
+            "traffic_source": tsrc,
+            "bytes_per_cell": round(bpc, 2),
+            "launches": nl,
         },
     }
     copy_gbs, _ = zdm.stream_copy_gbs("sme2", 1 << 30, 20)

@@ -141,9 +141,17 @@ int zdm_state_download_device(zdm_solver* s, double* dX, double* dA);
 /* Per-kernel HIP-event timing of zdm_step launches while enabled (ms, averaged per launch;
  * events are resolved without synchronising inside zdm_step).  on = 0 off, 1 every step, k > 1
  * every k-th step of zdm_step (a sample that leaves the other steps without event markers).
- * Kernel ids: 0 = prep (gradient/bounds/reconstruction), 1 = flux + stage update, 2 = post-step. */
+ * Kernel ids: 0 = prep (gradient/bounds/reconstruction), 1 = flux + stage update, 2 = post-step,
+ * 3 = the final stage's flux + stage update fused with the post-step (zdm_fuse_post). */
 int zdm_enable_timing(zdm_solver* s, int on);
 int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches);
+/* on = 1 (the default): a step of order >= 2 runs its final SSP stage and the post-step
+ * (constraints, split source, constraints, CFL) as one kernel, the new state staying in registers
+ * between them, where the step has no MOOD override, no derivative-aux refresh and no
+ * well-balanced term; 0: separate kernels.  Results are bitwise the same.  No counterpart in the
+ * reference (its MUSCLSolver runs TSStep, then the source and the constraints, as separate
+ * passes: MUSCLSolver.hpp:140-188). */
+int zdm_fuse_post(zdm_solver* s, int on);
 
 /* Measured HBM bandwidth of a plain device copy of `bytes` (read + write counted), averaged
  * over `iters` launches: the achievable-peak reference and the traffic-counter calibration. */

@@ -85,10 +85,14 @@ def test_kernel_timing_every_kth_step():
         s.initialize()
         s.enable_timing(stride)
         s.step(9)
-        ms, n = s.kernel_time(1)
-        expect = {0: 0, 1: 2 * 9, 3: 2 * 3}[stride]   # two flux launches per SSP-RK2 step
-        assert n == expect, (stride, n)
-        assert (ms > 0) == (expect > 0)
+        # two flux launches per SSP-RK2 step: the stage's k_flux (id 1) and the final stage
+        # fused with the post-step (id 3)
+        expect = {0: 0, 1: 9, 3: 3}[stride]
+        for kernel in (1, 3):
+            ms, n = s.kernel_time(kernel)
+            assert n == expect, (stride, kernel, n)
+            assert (ms > 0) == (expect > 0)
+        assert s.kernel_time(2)[1] == 0
         s.enable_timing(False)
         out.append(s.download()[0])
         s.close()

@@ -314,7 +314,7 @@ SOLVER_SYMBOLS = [
     "zdm_adjoint_dts", "zdm_adjoint_residuals", "zdm_adjoint_layout", "zdm_troubled_cells",
     "zdm_set_launch", "zdm_get_launch", "zdm_autotune", "zdm_adjoint_setup_chorin",
     "zdm_adjoint_krylov_its", "zdm_implicit_setup", "zdm_implicit_stats", "zdm_adjoint_stretch_tape",
-    "zdm_adjoint_forward_device", "zdm_adjoint_gradient_device", "zdm_solver_stream",
+    "zdm_adjoint_forward_device", "zdm_adjoint_gradient_device", "zdm_solver_stream", "zdm_fuse_post",
 ]
 
 
@@ -356,6 +356,7 @@ def model_lib(model):
     lib.zdm_cfl_dt.argtypes = [V, _c_dbl_p]
     lib.zdm_enable_timing.argtypes = [V, ctypes.c_int]
     lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
+    lib.zdm_fuse_post.argtypes = [V, ctypes.c_int]
     lib.zdm_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _c_dbl_p, _c_dbl_p]
     lib.zdm_last_error.restype = ctypes.c_char_p
     lib.zdm_batch_create.argtypes = [ctypes.POINTER(_ZdmMesh), _c_int_p, _c_dbl_p,
@@ -659,7 +660,15 @@ class Solver:
         """HIP-event kernel timing: False off, True every step, k > 1 every k-th step."""
         self._check(self.lib.zdm_enable_timing(self._h, int(on)))
 
+    def fuse_post(self, on=True):
+        """True (the default): the final SSP stage and the post-step run as one kernel where the
+        step allows it (no MOOD, no derivative-aux refresh, no well-balanced term); False:
+        separate kernels.  Bitwise the same results."""
+        self._check(self.lib.zdm_fuse_post(self._h, int(bool(on))))
+
     def kernel_time(self, kernel):
+        """(average ms, launches) of kernel id 0 prep, 1 flux + stage update, 2 post-step,
+        3 final stage fused with the post-step."""
         ms, n = ctypes.c_double(), ctypes.c_int()
         self._check(self.lib.zdm_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value

@@ -1311,6 +1311,67 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
     }
 }
 
+// The final SSP stage of a step and the post-stage of k_post<POST> (POST 1: constraints + CFL,
+// 2: constraints, source, constraints + CFL) in one pass: each cell's new state
+// X1 = al W + be F(W) + ga X0 stays in registers for the post-stage instead of a store, a load
+// and a second store through HBM; the post-stage's result goes to Xout (X0 = Xout in place for a
+// plain step).  The same operations on the same values as k_flux<MODE_FINAL> followed by
+// k_post<POST>, so the results are bit for bit theirs.  REC: the recording step of the adjoint,
+// which also keeps X1, and for POST 2 the aux before the source (A3) and the source's solution
+// (X3), as k_post_record does, and the step's dt in dt_log.  Not for the well-balanced cell term.
+template <int MAXF, int POST, bool MC, bool REC>
+__global__ __launch_bounds__(BLOCK) void k_flux_post(DevMesh m, Params P, const Ctrl* __restrict__ ctrl, int stage,
+                                                     const T* __restrict__ QF, const T* __restrict__ As,
+                                                     const T* __restrict__ W, const T* X0, T* Xout, T* A,
+                                                     T* __restrict__ partials, const T* __restrict__ G,
+                                                     T* __restrict__ X1, T* __restrict__ X3, T* __restrict__ A3,
+                                                     T* __restrict__ dt_log) {
+    static_assert(POST == 1 || POST == 2, "k_flux_post: POST 1 or 2");
+    const auto L = locate<MC>(m);
+    ctrl += L.m;
+    if (!ctrl->active) return;   // uniform over the workgroup (members own whole workgroups)
+    if (REC && dt_log && L.lb == 0 && threadIdx.x == 0) *dt_log = ctrl->dt;   // the step's dt
+    const int ld = m.ld;
+    T ratio = DBL_MAX;
+    auto body = [&](int c) -> T {
+        T q[NQ], a[NA];
+        {
+            const T time = ctrl->stage_t[stage];
+            T fc[NQ], ac[NA];
+            load_soa<NA>(As, ld, c, ac);
+            flux_cell_gather<MAXF>(m, P, c, time, ac, 0, nullptr, QF, As, fc, G);
+            // VecAXPBYPCZ(X, (s-1)/s, dt/s, 1/s, W, F) as stage_update<MODE_FINAL>
+            const T al = ctrl->al, be = ctrl->be, ga = ctrl->ga;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) q[i] = al * W[(size_t)i * ld + c] + be * fc[i] + ga * X0[(size_t)i * ld + c];
+        }
+        if constexpr (REC) store_soa<NQ>(X1, ld, c, q);
+        load_soa<NA>(A, ld, c, a);
+        update_state_cell(P, q, a);
+        if constexpr (POST == 2) {
+            if constexpr (REC) store_soa<NA>(A3, ld, c, a);
+            newton_cell(P, ctrl->dt, q, a);
+            if constexpr (REC) store_soa<NQ>(X3, ld, c, q);
+            update_state_cell(P, q, a);
+        }
+        store_soa<NQ>(Xout, ld, c, q);
+        store_soa<NA>(A, ld, c, a);
+        return cfl_ratio(P, q, a, m.rc[c]);
+    };
+    if constexpr (MC) {
+        for (int j = 0; j < m.cpt; ++j) {
+            const int c = L.cell(j);
+            if (c >= m.ncells) break;
+            const T r = body(c);
+            ratio = r < ratio ? r : ratio;
+        }
+    } else {
+        if (L.c < m.ncells) ratio = body(L.c);
+    }
+    ratio = block_min(ratio);
+    if (threadIdx.x == 0) partials[L.lb] = ratio;
+}
+
 // Post-stage of a step with the local MOOD override (MUSCLSolver.hpp:150-182): constraints on the
 // transport result X; a cell left with h < -1e-10 is replaced by its first-order forward-Euler
 // update from the saved start-of-step state Xs, X_save + dt F (F = k_flux's face loop over the
@@ -1660,8 +1721,9 @@ struct zdm_solver {
     std::vector<hipEvent_t> evpool;   // recorded in order; pending until resolved
     std::vector<int> evkind;          // kernel id that ends at event i (-1 = segment start)
     size_t evused;
-    double kt_ms[3];
-    int kt_n[3];
+    double kt_ms[4];
+    int kt_n[4];
+    bool fuse_post = true;            // zdm_fuse_post: the final stage and the post-step as k_flux_post
     ChorinHost* ch;                   // pressure split state (VAM models)
     int nmem;                         // ensemble members (0: one mesh)
     int nctrl;                        // control blocks: max(1, nmem)
@@ -1949,6 +2011,55 @@ void post(zdm_solver* s, int mode, T dt_override = 0) {
         post_t<true>(s, mode, dt_override);
     else
         post_t<false>(s, mode, dt_override);
+}
+
+// The final SSP stage fused with the post-stage of mode 1 (constraints + CFL) or 2 (constraints,
+// source, constraints + CFL): k_flux_post from the stage state W (face states QF, clamped aux As)
+// and the step's start state X0 into Xout, A.  rec (the adjoint's recording step, else null):
+// X1, X3, A3 and dt_log as k_post_record keeps them.
+struct FluxPostRec {
+    T *X1, *X3, *A3, *dt_log;
+};
+template <int MAXF, bool MC>
+void flux_post_t(zdm_solver* s, const Ctrl* ctrl, int mode, const T* QF, const T* As, const T* W, const T* X0,
+                 T* Xout, T* A, const FluxPostRec* rec) {
+    dim3 g(s->nblocks), b(s->bs);
+    const int stage = s->opt.ssp_stages - 1;
+    const T* G = grad_buffer(s);
+#define ZDM_FP(PO, RE)                                                                                       \
+    k_flux_post<MAXF, PO, MC, RE><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, W, X0, Xout, A,  \
+                                                          s->partials, G, rec ? rec->X1 : nullptr,            \
+                                                          rec ? rec->X3 : nullptr, rec ? rec->A3 : nullptr,   \
+                                                          rec ? rec->dt_log : nullptr)
+    if (mode == 2) {
+        if (rec)
+            ZDM_FP(2, true);
+        else
+            ZDM_FP(2, false);
+    } else {
+        if (rec)
+            ZDM_FP(1, true);
+        else
+            ZDM_FP(1, false);
+    }
+#undef ZDM_FP
+}
+template <int MAXF>
+void flux_post_m(zdm_solver* s, const Ctrl* ctrl, int mode, const T* QF, const T* As, const T* W, const T* X0,
+                 T* Xout, T* A, const FluxPostRec* rec) {
+    if (s->dm.cpt > 1)
+        flux_post_t<MAXF, true>(s, ctrl, mode, QF, As, W, X0, Xout, A, rec);
+    else
+        flux_post_t<MAXF, false>(s, ctrl, mode, QF, As, W, X0, Xout, A, rec);
+}
+void flux_post(zdm_solver* s, const Ctrl* ctrl, int mode, const T* QF, const T* As, const T* W, const T* X0,
+               T* Xout, T* A, const FluxPostRec* rec = nullptr) {
+    if (s->maxf <= 2)
+        flux_post_m<2>(s, ctrl, mode, QF, As, W, X0, Xout, A, rec);
+    else if (s->maxf == 3)
+        flux_post_m<3>(s, ctrl, mode, QF, As, W, X0, Xout, A, rec);
+    else
+        flux_post_m<4>(s, ctrl, mode, QF, As, W, X0, Xout, A, rec);
 }
 
 void finalize(zdm_solver* s, int what) {
@@ -2392,7 +2503,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     s->timing_stride = 0;
     s->timing_step = 0;
     s->evused = 0;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
         s->kt_ms[i] = 0;
         s->kt_n[i] = 0;
     }
@@ -2621,6 +2732,13 @@ static void enqueue_step(zdm_solver* s) {
         }
         prep(s, s->ctrl, s->W, s->A);
         record(s, 0);
+        if (s->fuse_post && !mood && !s->refresh && !s->opt.well_balanced) {
+            // final stage and post-step in one pass (k_flux_post): bitwise the separate kernels
+            flux_post(s, s->ctrl, s->opt.source ? 2 : 1, s->QF, s->As, s->W, s->X, s->X, s->A);
+            record(s, 3);
+            finalize(s, 2);
+            return;
+        }
         flux(s, s->ctrl, MODE_FINAL, st - 1, s->W, s->X, s->A);
         record(s, 1);
     }
@@ -2693,15 +2811,21 @@ int zdm_enable_timing(zdm_solver* s, int on) {
     s->timing_stride = on == 0 ? 0 : on < 0 ? 1 : on;
     s->timing_step = 0;
     s->timing = on != 0;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
         s->kt_ms[i] = 0;
         s->kt_n[i] = 0;
     }
     return 0;
 }
 
+int zdm_fuse_post(zdm_solver* s, int on) {
+    if (!s) return fail("null solver");
+    s->fuse_post = on != 0;
+    return 0;
+}
+
 int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches) {
-    if (!s || kernel < 0 || kernel > 2) return fail("bad kernel id");
+    if (!s || kernel < 0 || kernel > 3) return fail("bad kernel id");
     if (resolve_events(s)) return -1;
     *launches = s->kt_n[kernel];
     *avg_ms = s->kt_n[kernel] ? s->kt_ms[kernel] / s->kt_n[kernel] : 0.0;
