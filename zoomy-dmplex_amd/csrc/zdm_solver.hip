@@ -4,7 +4,9 @@
 // reference builds solver_cpu on its generated headers.  The headers are consumed unchanged:
 // they are compiled for host and device through clang's force_cuda_host_device pragma.
 //
-// Device design (one GPU; FP64 throughout; no atomics; results independent of scheduling):
+// Device design (one GPU; FP64 throughout; results independent of scheduling: no floating-point
+// atomics; the only atomics are integer counters, the MOOD troubled-cell count and the
+// last-arrival counter of k_finalize_xcd, whose values do not depend on the order of arrival):
 //   * structure-of-arrays state x[i*ld + c], cells renumbered by reverse Cuthill-McKee;
 //   * k_prep  (per cell): wet/dry clamp + aux, face-neighbour bounds, Green-Gauss or
 //     least-squares gradient, limited MUSCL reconstruction of every face of the cell into a
@@ -14,7 +16,9 @@
 //     by the volume, with the SSP-RK stage update fused into the epilogue;
 //   * k_post  (per cell): constraints -> implicit source Newton -> constraints -> local CFL
 //     ratio, reduced per workgroup; k_finalize turns the partial minima into the next dt and
-//     keeps time / dt on the device, so a time step needs no host synchronisation.
+//     keeps time / dt on the device, so a time step needs no host synchronisation;
+//   * k_flux_post: the final stage's k_flux and k_post in one pass, the new state in registers
+//     between them (the default form of a step).
 // Workgroups are remapped so that each XCD (own L2) works on a contiguous range of cells.
 #include <hip/hip_runtime.h>
 
