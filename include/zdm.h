@@ -125,6 +125,15 @@ int zdm_synchronize(zdm_solver* s);
  * TransportStep.hpp:325-357) on host arrays; A is the scratch aux input (may be NULL). */
 int zdm_transport_rhs(zdm_solver* s, double t, const double* X, const double* A, double* F);
 
+/* The same right-hand side on AoS float64 device buffers (X: ncells*n_q, A: ncells*n_aux or
+ * NULL, F: ncells*n_q), ordered on `stream` (a hipStream_t; the solver's own stream when it is
+ * zdm_solver_stream's): the call's work starts after what `stream` holds and later work on
+ * `stream` sees F.  It does not synchronise the host.  Bitwise zdm_transport_rhs's result.
+ * Replaces TransportStep::FormRHS(time, X, F) (TransportStep.hpp:325) where X and F live on
+ * the device; not with the derivative-aux refresh (that host loop synchronises). */
+int zdm_transport_rhs_device(zdm_solver* s, double t, const double* dX, const double* dA, double* dF,
+                             void* stream);
+
 /* SourceStep::Solve(dt, X, A) on the resident state (SourceStep.hpp:72-122). */
 int zdm_source_step(zdm_solver* s, double dt);
 

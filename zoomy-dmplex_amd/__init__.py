@@ -315,6 +315,7 @@ SOLVER_SYMBOLS = [
     "zdm_set_launch", "zdm_get_launch", "zdm_autotune", "zdm_adjoint_setup_chorin",
     "zdm_adjoint_krylov_its", "zdm_implicit_setup", "zdm_implicit_stats", "zdm_adjoint_stretch_tape",
     "zdm_adjoint_forward_device", "zdm_adjoint_gradient_device", "zdm_solver_stream", "zdm_fuse_post",
+    "zdm_transport_rhs_device",
 ]
 
 
@@ -357,6 +358,7 @@ def model_lib(model):
     lib.zdm_enable_timing.argtypes = [V, ctypes.c_int]
     lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
     lib.zdm_fuse_post.argtypes = [V, ctypes.c_int]
+    lib.zdm_transport_rhs_device.argtypes = [V, ctypes.c_double, V, V, V, V]
     lib.zdm_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _c_dbl_p, _c_dbl_p]
     lib.zdm_last_error.restype = ctypes.c_char_p
     lib.zdm_batch_create.argtypes = [ctypes.POINTER(_ZdmMesh), _c_int_p, _c_dbl_p,
@@ -644,6 +646,16 @@ class Solver:
             Ap = _dp(A)
         self._check(self.lib.zdm_transport_rhs(self._h, float(t), _dp(X), Ap, _dp(F)))
         return F
+
+    def transport_rhs_device(self, X_ptr, F_ptr, A_ptr=None, t=0.0, stream=None):
+        """The right-hand side on device buffers (integer addresses of float64 AoS arrays, e.g.
+        torch tensors' data_ptr()), ordered on `stream` (a hipStream_t as an int, e.g.
+        torch.cuda.current_stream().cuda_stream; None: the solver's stream).  No host
+        synchronisation."""
+        st = self.stream_handle() if stream is None else int(stream)
+        self._check(self.lib.zdm_transport_rhs_device(self._h, float(t), ctypes.c_void_p(int(X_ptr)),
+                                                      ctypes.c_void_p(int(A_ptr)) if A_ptr else None,
+                                                      ctypes.c_void_p(int(F_ptr)), ctypes.c_void_p(st)))
 
     def source_step(self, dt):
         self._check(self.lib.zdm_source_step(self._h, float(dt)))

@@ -1728,6 +1728,7 @@ struct zdm_solver {
     double kt_ms[4];
     int kt_n[4];
     bool fuse_post = true;            // zdm_fuse_post: the final stage and the post-step as k_flux_post
+    hipEvent_t ev_caller = nullptr;   // orders the device-pointer entry points on a caller's stream
     ChorinHost* ch;                   // pressure split state (VAM models)
     int nmem;                         // ensemble members (0: one mesh)
     int nctrl;                        // control blocks: max(1, nmem)
@@ -2555,6 +2556,7 @@ int zdm_solver_destroy(zdm_solver* s) {
     if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
     if (s->graph) hipGraphDestroy(s->graph);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
+    if (s->ev_caller) hipEventDestroy(s->ev_caller);
     chorin_free(s->ch);
     adjoint_free(s->adj);
     implicit_free(s->imp);
@@ -2901,6 +2903,53 @@ int zdm_transport_rhs(zdm_solver* s, double t, const double* X, const double* A,
     flux(s, s->ctrl_rhs, MODE_RHS, 0, s->W, s->F, s->Atmp);
     HIPCHK(hipGetLastError());
     return download_soa(s, s->F, F, NQ, false);
+}
+
+// the right-hand side's control blocks at time t, on the stream (set_ctrl_rhs_time without the
+// host copy and synchronisation)
+__global__ void k_set_rhs_time(Ctrl* c, int n, T t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ctrl z;
+    z.t = t;
+    z.dt = z.dt_cfl = z.dt_query = z.a_stage = z.al = z.be = z.ga = 0;
+    for (int k = 0; k < 16; ++k) z.stage_t[k] = 0;
+    z.stage_t[0] = t;
+    z.active = 1;
+    z.steps = z.finite = z.troubled = 0;
+    c[i] = z;
+}
+
+int zdm_transport_rhs_device(zdm_solver* s, double t, const double* dX, const double* dA, double* dF,
+                             void* stream) {
+    if (!s || !dX || !dF) return fail("zdm_transport_rhs_device: null argument");
+    if (s->refresh) return fail("zdm_transport_rhs_device: not for the derivative-aux refresh (it synchronises); "
+                                "use zdm_transport_rhs");
+    hipStream_t cs = (hipStream_t)stream, own = s->stream;
+    const bool other = cs != own;
+    if (other) {
+        // the call's work starts after what the caller's stream holds
+        if (!s->ev_caller) HIPCHK(hipEventCreateWithFlags(&s->ev_caller, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(s->ev_caller, cs));
+        HIPCHK(hipStreamWaitEvent(own, s->ev_caller, 0));
+    }
+    const unsigned g = (unsigned)((s->ncells + 255) / 256);
+    k_aos_to_soa<<<g, 256, 0, own>>>(dX, s->W, s->ncells, NQ, s->ld);
+    if (dA)
+        k_aos_to_soa<<<g, 256, 0, own>>>(dA, s->Atmp, s->ncells, NA, s->ld);
+    else if (fill_zero(s, s->Atmp, (size_t)NA * s->ld))
+        return -1;
+    k_set_rhs_time<<<(s->nctrl + 63) / 64, 64, 0, own>>>(s->ctrl_rhs, s->nctrl, (T)t);
+    prep(s, s->ctrl_rhs, s->W, s->Atmp);
+    flux(s, s->ctrl_rhs, MODE_RHS, 0, s->W, s->F, s->Atmp);
+    k_soa_to_aos<<<g, 256, 0, own>>>(s->F, dF, s->ncells, NQ, s->ld);
+    HIPCHK(hipGetLastError());
+    if (other) {
+        // later work on the caller's stream sees F
+        HIPCHK(hipEventRecord(s->ev_caller, own));
+        HIPCHK(hipStreamWaitEvent(cs, s->ev_caller, 0));
+    }
+    return 0;
 }
 
 int zdm_source_step(zdm_solver* s, double dt) {
