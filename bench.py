@@ -827,7 +827,9 @@ def run_c5_fd(zdm, cases, args):
     """Central differences of the device's own frozen-dt forward run against
     <gradient, direction> on --directions random directions (white and smooth), for each FD
     step in --fd-steps (the configuration: 1e-6), at each horizon in --horizons (steps), in
-    both limiter modes, with the configured layout."""
+    both limiter modes, with the configured layout.  --frozen-map adds, for the frozen limiter,
+    the differences of the map that gradient differentiates: the forward with every face's
+    limiter value held at the unperturbed run's (zdm_adjoint_forward_frozen_alpha)."""
     horizons = [int(v) for v in args.horizons.split(",")]
     epss = [float(v) for v in args.fd_steps.split(",")]
     obs_every = 5
@@ -838,12 +840,18 @@ def run_c5_fd(zdm, cases, args):
     for T in horizons:
         oe = min(obs_every, T)
         obs = c5_observations(s, T, gauges, oe, Xt, At)
-        for exact in (True, False):
+        modes = [(True, "plain"), (False, "plain")] + ([(False, "frozen")] if args.frozen_map else [])
+        for exact, fmap in modes:
             s.adjoint_setup(T, gauges=gauges, obs_every=oe, obs=obs, checkpoint_every=c5_default_layout(T),
                             exact_limiter=exact)
             loss = s.adjoint_forward(Xg, Ag)
             dts = s.adjoint_dts().copy()
             gX, gA = s.adjoint_gradient()
+            if fmap == "frozen":
+                s.adjoint_forward_frozen_alpha(Xg, Ag, dts=dts, record=True)
+                fwd = lambda X_, A_: s.adjoint_forward_frozen_alpha(X_, A_, dts=dts)
+            else:
+                fwd = lambda X_, A_: s.adjoint_forward(X_, A_, dts=dts)
             for kind, dirs in dirsets.items():
                 for eps in epss:
                     errs = []
@@ -853,11 +861,12 @@ def run_c5_fd(zdm, cases, args):
                         Xm[:, 0] -= eps * db
                         Ap[:, 1] += eps * dn
                         Am[:, 1] -= eps * dn
-                        fd = (s.adjoint_forward(Xp, Ap, dts=dts) - s.adjoint_forward(Xm, Am, dts=dts)) / (2 * eps)
+                        fd = (fwd(Xp, Ap) - fwd(Xm, Am)) / (2 * eps)
                         ad = float(np.dot(gX[:, 0], db) + np.dot(gA[:, 1], dn))
                         errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
                     errs = np.array(errs)
-                    row = {"steps": T, "exact_limiter": exact, "directions": kind, "fd_step": eps, "loss": loss,
+                    row = {"steps": T, "exact_limiter": exact, "fd_map": fmap, "directions": kind, "fd_step": eps,
+                           "loss": loss,
                            "grad_norm": float(np.sqrt(np.sum(gX[:, 0] ** 2) + np.sum(gA[:, 1] ** 2))),
                            "rel_err_median": float(np.median(errs)), "rel_err_max": float(np.max(errs)),
                            "within_1e-6": int(np.sum(errs <= 1e-6)), "count": len(errs)}
@@ -902,6 +911,8 @@ def main():
     ap.add_argument("--horizons", default="1,2,5,10,25,50,100,200,500", help="c5fd: trajectory lengths")
     ap.add_argument("--fd-steps", default="1e-6", help="c5fd: central-difference steps")
     ap.add_argument("--directions", type=int, default=64, help="c5fd: random directions")
+    ap.add_argument("--frozen-map", action="store_true",
+                    help="c5fd: also difference the frozen-limiter map for the frozen-limiter gradient")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: c2 200, c3 10, c4 300, c5 500, c3adj 10)")

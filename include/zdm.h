@@ -320,6 +320,15 @@ int zdm_adjoint_stretch_tape(zdm_solver* s, int steps, int buffers, int* steps_o
  * re-run, e.g. for finite differences).  Returns the loss. */
 int zdm_adjoint_forward(zdm_solver* s, const double* X0, const double* A0, const double* dts,
                         double* loss);
+/* The forward map the frozen-limiter gradient (exact_limiter = 0) differentiates, for checking
+ * it by differences: a forward run of the calibration problem whose face limiter values are
+ * stored (record = 1: every step and stage; dts may be NULL for the CFL steps) or taken from the
+ * last recording (record = 0; dts required, the recording's sequence).  The loss is returned as
+ * zdm_adjoint_forward's; the run overwrites the forward a gradient would reverse (call
+ * zdm_adjoint_forward again before zdm_adjoint_gradient).  Order 2 only.  No counterpart in the
+ * reference (which has no adjoint). */
+int zdm_adjoint_forward_frozen_alpha(zdm_solver* s, const double* X0, const double* A0, const double* dts,
+                                     int record, double* loss);
 /* Gradient of the loss of the last forward run: gX0 (ncells*n_q), gA0 (ncells*n_aux);
  * either may be NULL. */
 int zdm_adjoint_gradient(zdm_solver* s, double* gX0, double* gA0);

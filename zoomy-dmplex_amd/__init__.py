@@ -315,7 +315,7 @@ SOLVER_SYMBOLS = [
     "zdm_set_launch", "zdm_get_launch", "zdm_autotune", "zdm_adjoint_setup_chorin",
     "zdm_adjoint_krylov_its", "zdm_implicit_setup", "zdm_implicit_stats", "zdm_adjoint_stretch_tape",
     "zdm_adjoint_forward_device", "zdm_adjoint_gradient_device", "zdm_solver_stream", "zdm_fuse_post",
-    "zdm_transport_rhs_device",
+    "zdm_transport_rhs_device", "zdm_adjoint_forward_frozen_alpha",
 ]
 
 
@@ -359,6 +359,7 @@ def model_lib(model):
     lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
     lib.zdm_fuse_post.argtypes = [V, ctypes.c_int]
     lib.zdm_transport_rhs_device.argtypes = [V, ctypes.c_double, V, V, V, V]
+    lib.zdm_adjoint_forward_frozen_alpha.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]
     lib.zdm_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _c_dbl_p, _c_dbl_p]
     lib.zdm_last_error.restype = ctypes.c_char_p
     lib.zdm_batch_create.argtypes = [ctypes.POINTER(_ZdmMesh), _c_int_p, _c_dbl_p,
@@ -789,6 +790,26 @@ class Solver:
             dp_ = _dp(dts)
         L = ctypes.c_double()
         self._check(self.lib.zdm_adjoint_forward(self._h, _dp(X0), Ap, dp_, ctypes.byref(L)))
+        return L.value
+
+    def adjoint_forward_frozen_alpha(self, X0, A0=None, dts=None, record=False):
+        """The map the frozen-limiter gradient differentiates: a forward run whose face limiter
+        values are recorded (record=True) or replayed from the last recording (dts required),
+        for central differences.  Overwrites the forward a gradient would reverse."""
+        X0 = np.ascontiguousarray(X0, dtype=np.float64).reshape(self.mesh.ncells, self.info.n_q)
+        Ap = None
+        if A0 is not None:
+            A0 = np.ascontiguousarray(A0, dtype=np.float64).reshape(self.mesh.ncells, self.info.n_aux)
+            Ap = _dp(A0)
+        dp_ = None
+        if dts is not None:
+            dts = np.ascontiguousarray(dts, dtype=np.float64)
+            if dts.size != self._adj[0]:
+                raise ZdmError("adjoint_forward_frozen_alpha: dts must hold nsteps values")
+            dp_ = _dp(dts)
+        L = ctypes.c_double()
+        self._check(self.lib.zdm_adjoint_forward_frozen_alpha(self._h, _dp(X0), Ap, dp_, int(bool(record)),
+                                                              ctypes.byref(L)))
         return L.value
 
     def adjoint_gradient(self):

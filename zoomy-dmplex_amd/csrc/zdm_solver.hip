@@ -1729,6 +1729,8 @@ struct zdm_solver {
     int kt_n[4];
     bool fuse_post = true;            // zdm_fuse_post: the final stage and the post-step as k_flux_post
     hipEvent_t ev_caller = nullptr;   // orders the device-pointer entry points on a caller's stream
+    int alpha_mode = 0;               // prep: 0 as configured, 1 store every face's limiter value at
+    T* alpha_io = nullptr;            // alpha_io, 2 take them from it (advanced one stage per call)
     ChorinHost* ch;                   // pressure split state (VAM models)
     int nmem;                         // ensemble members (0: one mesh)
     int nctrl;                        // control blocks: max(1, nmem)
@@ -1936,6 +1938,25 @@ void prep_record(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* 
 }
 
 // face states / clamped aux into (QF, As): the solver's own buffers unless given
+// k_prep with the face limiter values stored in (alpha_mode 1) or taken from (2) s->alpha_io:
+// one mesh, order 2, the configured gradient
+template <int MAXF>
+void prep_alpha_io(zdm_solver* s, const T* Xin, const T* Ag, T* As, T* QF) {
+    dim3 g((s->ncells + BLOCK - 1) / BLOCK), b(BLOCK);
+    const int lim = s->opt.use_limiters;
+    if (s->opt.gradient == 1) {
+        if (s->alpha_mode == 1)
+            k_prep_alpha<MAXF, 2, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, Xin, Ag, As, QF, lim, s->alpha_io);
+        else
+            k_prep_alpha<MAXF, 2, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, Xin, Ag, As, QF, lim, s->alpha_io);
+    } else {
+        if (s->alpha_mode == 1)
+            k_prep_alpha<MAXF, 1, 1><<<g, b, 0, s->stream>>>(s->dm, s->P, Xin, Ag, As, QF, lim, s->alpha_io);
+        else
+            k_prep_alpha<MAXF, 1, 2><<<g, b, 0, s->stream>>>(s->dm, s->P, Xin, Ag, As, QF, lim, s->alpha_io);
+    }
+}
+
 void prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* QF = nullptr, T* As = nullptr) {
     QF = QF ? QF : s->QF;
     As = As ? As : s->As;
@@ -1949,6 +1970,17 @@ void prep(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T* QF = nu
             return;
         }
         Ag = s->Aref;
+    }
+    if (s->alpha_mode) {
+        // the limiter values of this stage stored in (1) or taken from (2) alpha_io
+        if (s->maxf <= 2)
+            prep_alpha_io<2>(s, Xin, Ag, As, QF);
+        else if (s->maxf == 3)
+            prep_alpha_io<3>(s, Xin, Ag, As, QF);
+        else
+            prep_alpha_io<4>(s, Xin, Ag, As, QF);
+        s->alpha_io += (size_t)(s->maxf <= 2 ? 2 : s->maxf) * s->ld;
+        return;
     }
     if (s->maxf <= 2)
         launch_prep<2>(s, ctrl, Xin, Ag, As, QF);
@@ -3454,6 +3486,14 @@ int zdm_adjoint_forward(zdm_solver* s, const double* X0, const double* A0, const
     if (!s || !X0) return fail("zdm_adjoint_forward: null argument");
     if (!s->adj) return fail("zdm_adjoint_forward: call zdm_adjoint_setup first");
     return adjoint_forward(s, X0, A0, dts, loss);
+}
+
+int zdm_adjoint_forward_frozen_alpha(zdm_solver* s, const double* X0, const double* A0, const double* dts,
+                                     int record, double* loss) {
+    if (!s || !X0) return fail("zdm_adjoint_forward_frozen_alpha: null argument");
+    if (!s->adj) return fail("zdm_adjoint_forward_frozen_alpha: call zdm_adjoint_setup first");
+    if (s->dm.nmem > 0) return fail("zdm_adjoint_forward_frozen_alpha: not for an ensemble");
+    return adjoint_forward_frozen_alpha(s, X0, A0, dts, record, loss);
 }
 
 int zdm_adjoint_gradient(zdm_solver* s, double* gX0, double* gA0) {
