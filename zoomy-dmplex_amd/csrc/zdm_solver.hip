@@ -2458,7 +2458,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
         return fail("hipStreamCreate failed");
     }
     if (hipMalloc(&s->arena, s->arena_bytes) != hipSuccess) {
-        hipStreamDestroy(s->stream);
+        (void)hipStreamDestroy(s->stream);
         delete s;
         return fail("hipMalloc of the solver arena failed");
     }
@@ -2504,8 +2504,8 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
         e = e != hipSuccess ? e : up(oCo, coff.data(), sizeof(int) * coff.size());
     }
     if (e != hipSuccess) {
-        hipFree(s->arena);
-        hipStreamDestroy(s->stream);
+        (void)hipFree(s->arena);
+        (void)hipStreamDestroy(s->stream);
         delete s;
         return fail(std::string("device upload failed: ") + hipGetErrorString(e));
     }
@@ -2546,7 +2546,10 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     }
     std::vector<Ctrl> c0(s->nctrl);
     std::memset(c0.data(), 0, sizeof(Ctrl) * c0.size());
-    hipMemcpy(s->ctrl, c0.data(), sizeof(Ctrl) * c0.size(), hipMemcpyHostToDevice);
+    if (hipMemcpy(s->ctrl, c0.data(), sizeof(Ctrl) * c0.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        zdm_solver_destroy(s);
+        return fail("device upload of the control blocks failed");
+    }
 #ifdef ZOOMY_MODEL_IS_VAM
     if (nmem == 0 && chorin_build(s, mesh, krylov_defaults())) {
         const std::string msg = g_err;
@@ -2584,16 +2587,17 @@ int zdm_batch_create(const zdm_mesh* mesh, const int* face_bc, const double* par
 
 int zdm_solver_destroy(zdm_solver* s) {
     if (!s) return 0;
-    hipStreamSynchronize(s->stream);
-    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-    if (s->graph) hipGraphDestroy(s->graph);
-    for (hipEvent_t e : s->evpool) hipEventDestroy(e);
-    if (s->ev_caller) hipEventDestroy(s->ev_caller);
+    // teardown: errors left by earlier work are not this call's to report
+    (void)hipStreamSynchronize(s->stream);
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
+    if (s->ev_caller) (void)hipEventDestroy(s->ev_caller);
     chorin_free(s->ch);
     adjoint_free(s->adj);
     implicit_free(s->imp);
-    hipFree(s->arena);
-    hipStreamDestroy(s->stream);
+    (void)hipFree(s->arena);
+    (void)hipStreamDestroy(s->stream);
     delete s;
     return 0;
 }
@@ -2683,7 +2687,7 @@ static void record(zdm_solver* s, int kind) {
         s->evkind.push_back(-1);
     }
     s->evkind[s->evused] = kind;
-    hipEventRecord(s->evpool[s->evused], s->stream);
+    (void)hipEventRecord(s->evpool[s->evused], s->stream);
     s->evused++;
 }
 
@@ -3367,8 +3371,8 @@ int zdm_set_launch(zdm_solver* s, int block_size, int cells_per_thread) {
     s->dm.cpt = cells_per_thread;
     s->nblocks = nb;
     // a captured step holds the old launch shapes
-    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-    if (s->graph) hipGraphDestroy(s->graph);
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
     s->graph_exec = nullptr;
     s->graph = nullptr;
     return 0;
@@ -3438,9 +3442,9 @@ int zdm_autotune(zdm_solver* s, int steps, int* block_size, int* cells_per_threa
     if (rc == 0) rc = zdm_set_launch(s, cand[best][0], cand[best][1]);
     if (rc == 0 && (copy3(false) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess))
         rc = fail("zdm_autotune: restoring the state failed");
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipFree(save);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(save);
     s->timing = timing;
     if (rc) return rc;
     if (block_size) *block_size = cand[best][0];
