@@ -991,6 +991,7 @@ def main():
     Xg, _ = s.download()
     if not np.all(np.isfinite(Xg)):
         raise RuntimeError("non-finite state after the benchmark")
+    nontemporal = s.flux_nontemporal() and shape[1] == 1   # the by-size default the run used
 
     ncells = mesh.ncells
     F = mesh.nfaces / ncells
@@ -1031,6 +1032,7 @@ def main():
                         "(%d cells), MUSCL + Barth-Jespersen, SSP-RK2, CFL 0.45" % (n, n, ncells),
             "cells": ncells,
             "cell_order": args.order,
+            "flux_nontemporal": nontemporal,
             "n_q": info.n_q,
             "faces_per_cell": round(F, 4),
             "step_bytes_per_cell": round(step_bytes_per_cell(info.n_q, F), 1),

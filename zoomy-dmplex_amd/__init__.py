@@ -315,7 +315,7 @@ SOLVER_SYMBOLS = [
     "zdm_set_launch", "zdm_get_launch", "zdm_autotune", "zdm_adjoint_setup_chorin",
     "zdm_adjoint_krylov_its", "zdm_implicit_setup", "zdm_implicit_stats", "zdm_adjoint_stretch_tape",
     "zdm_adjoint_forward_device", "zdm_adjoint_gradient_device", "zdm_solver_stream", "zdm_fuse_post",
-    "zdm_transport_rhs_device", "zdm_adjoint_forward_frozen_alpha",
+    "zdm_transport_rhs_device", "zdm_adjoint_forward_frozen_alpha", "zdm_flux_nontemporal",
 ]
 
 
@@ -358,6 +358,7 @@ def model_lib(model):
     lib.zdm_enable_timing.argtypes = [V, ctypes.c_int]
     lib.zdm_kernel_time.argtypes = [V, ctypes.c_int, _c_dbl_p, _c_int_p]
     lib.zdm_fuse_post.argtypes = [V, ctypes.c_int]
+    lib.zdm_flux_nontemporal.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.zdm_transport_rhs_device.argtypes = [V, ctypes.c_double, V, V, V, V]
     lib.zdm_adjoint_forward_frozen_alpha.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, _c_dbl_p]
     lib.zdm_stream_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _c_dbl_p, _c_dbl_p]
@@ -678,6 +679,15 @@ class Solver:
         step allows it (no MOOD, no derivative-aux refresh, no well-balanced term); False:
         separate kernels.  Bitwise the same results."""
         self._check(self.lib.zdm_fuse_post(self._h, int(bool(on))))
+
+    def flux_nontemporal(self, mode=-1):
+        """Cache policy of the stage and final flux launches: 1 non-temporal accesses for their
+        read-once / write-once streams, 0 ordinary ones, -1 (the default) by mesh size (on where
+        one flux pass exceeds 256 MiB).  Returns whether the non-temporal form is now in use.
+        Bitwise the same results either way."""
+        active = ctypes.c_int()
+        self._check(self.lib.zdm_flux_nontemporal(self._h, int(mode), ctypes.byref(active)))
+        return bool(active.value)
 
     def kernel_time(self, kernel):
         """(average ms, launches) of kernel id 0 prep, 1 flux + stage update, 2 post-step,

@@ -313,6 +313,37 @@ __device__ __forceinline__ void store_soa(T* __restrict__ a, int ld, int c, cons
 #pragma unroll
     for (int i = 0; i < N; ++i) a[(size_t)i * ld + c] = v[i];
 }
+// The flux kernels' read-once and write-once streams (the stage state, Q^n, the output state
+// and aux, the cell's own slot topology and volume), non-temporal when NT: on a mesh whose flux
+// pass is far larger than the last-level cache they then do not displace the face states and
+// face records that a cell's neighbours read again in the same launch (C2: 551 -> 512 B/cell
+// moved by the stage launch).  On a smaller mesh the same hints keep data out of the 256 MB
+// Infinity Cache that the next kernel would have found there, and cost time, so the solver
+// picks them by size (zdm_flux_nontemporal).
+template <bool NT, typename S>
+__device__ __forceinline__ S ldx(const S* p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+template <bool NT, typename S>
+__device__ __forceinline__ void stx(S v, S* p) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+template <bool NT, int N>
+__device__ __forceinline__ void load_soa_x(const T* __restrict__ a, int ld, int c, T* v) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = ldx<NT>(a + (size_t)i * ld + c);
+}
+template <bool NT, int N>
+__device__ __forceinline__ void store_soa_x(T* __restrict__ a, int ld, int c, const T* v) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) stx<NT>(v[i], a + (size_t)i * ld + c);
+}
 
 // TransportStep::UpdateState per cell (TransportStep.hpp:210-227)
 __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
@@ -798,8 +829,9 @@ __device__ __forceinline__ void flux_face(const DevMesh& m, const Params& P, int
     }
 }
 
+template <bool NT = false>
 __device__ __forceinline__ void flux_divide_volume(const DevMesh& m, int c, T* fc) {
-    const T vol = m.vol[c];
+    const T vol = ldx<NT>(m.vol + c);
     if (vol > 1e-15) {
 #pragma unroll
         for (int i = 0; i < NQ; ++i) fc[i] /= vol;
@@ -849,7 +881,7 @@ struct FaceIn {
     T gn[NGD];   // the neighbour's cell gradient (diffusive models only)
 };
 
-template <int MAXF>
+template <int MAXF, bool NT = false>
 __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params& P, int c, T time, const T* ac,
                                                  int wb, const T* qcell, const T* __restrict__ QF,
                                                  const T* __restrict__ As, T* fc, const T* __restrict__ G = nullptr) {
@@ -862,8 +894,8 @@ __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params&
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) {
         const size_t e = 2 * ((size_t)k * ld + c);
-        const int nm = m.ctopo[e + 1];
-        fk[k] = m.ctopo[e];
+        const int nm = ldx<NT>(m.ctopo + e + 1);
+        fk[k] = ldx<NT>(m.ctopo + e);
         nk[k] = nm >= 0 ? (nm & 0x0FFFFFFF) : -1;
         mk[k] = nm >= 0 ? (nm >> 28) : 0;
     }
@@ -898,34 +930,36 @@ __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params&
             flux_face(m, P, fk[k], nk[k], mk[k], in.area, in.nh, in.xf, in.qo, in.qn, in.an, ac, time, wb, qcell,
                       fc, HasDiffusiveFlux<T>::value && G ? go : nullptr, in.gn);
     }
-    flux_divide_volume(m, c, fc);
+    flux_divide_volume<NT>(m, c, fc);
 }
 
 // Out = the stage update of the cell from its right-hand side fc (MODE_RHS: fc itself)
-template <int MODE>
+template <int MODE, bool NT = false>
 __device__ __forceinline__ void stage_update(const Ctrl* ctrl, int c, int ld, const T* fc, const T* In, T* Out,
                                              const T* X0) {
     if constexpr (MODE == MODE_RHS) {
-        store_soa<NQ>(Out, ld, c, fc);
+        store_soa_x<NT, NQ>(Out, ld, c, fc);
     } else if constexpr (MODE == MODE_EULER || MODE == MODE_STAGE) {
         // VecAXPY(u, a, F): u += a*F   (Euler: a = dt; SSP stage: a = dt/(s-1))
         const T a = ctrl->a_stage;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) Out[(size_t)i * ld + c] = In[(size_t)i * ld + c] + a * fc[i];
+        for (int i = 0; i < NQ; ++i) stx<NT>(ldx<NT>(In + (size_t)i * ld + c) + a * fc[i], Out + (size_t)i * ld + c);
     } else if constexpr (MODE == MODE_FINAL) {
         // VecAXPBYPCZ(X, (s-1)/s, dt/s, 1/s, W, F)
         const T al = ctrl->al, be = ctrl->be, ga = ctrl->ga;
 #pragma unroll
         for (int i = 0; i < NQ; ++i)
-            Out[(size_t)i * ld + c] = al * In[(size_t)i * ld + c] + be * fc[i] + ga * X0[(size_t)i * ld + c];
+            stx<NT>(al * ldx<NT>(In + (size_t)i * ld + c) + be * fc[i] + ga * ldx<NT>(X0 + (size_t)i * ld + c),
+                    Out + (size_t)i * ld + c);
     } else {
         const T dt = ctrl->dt;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) Out[(size_t)i * ld + c] = X0[(size_t)i * ld + c] + dt * (ark2::B3 * fc[i]);
+        for (int i = 0; i < NQ; ++i)
+            stx<NT>(ldx<NT>(X0 + (size_t)i * ld + c) + dt * (ark2::B3 * fc[i]), Out + (size_t)i * ld + c);
     }
 }
 
-template <int MAXF, int MODE, bool MC, typename TT = T>
+template <int MAXF, int MODE, bool MC, typename TT = T, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 int stage, const T* __restrict__ QF,
                                                 const T* __restrict__ As, const T* In, T* Out,
@@ -950,8 +984,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh m, Params P, const Ctrl*
                 for (int i = 0; i < NQ; ++i) qcell[i] = rq[i];
             }
         }
-        flux_cell_gather<MAXF>(m, P, c, time, ac, wb, qcell, QF, As, fc, G);
-        stage_update<MODE>(ctrl, c, ld, fc, In, Out, X0);
+        flux_cell_gather<MAXF, NT>(m, P, c, time, ac, wb, qcell, QF, As, fc, G);
+        stage_update<MODE, NT>(ctrl, c, ld, fc, In, Out, X0);
     };
     if constexpr (MC) {
         for (int j = 0; j < m.cpt; ++j) {
@@ -1323,7 +1357,7 @@ __global__ __launch_bounds__(BLOCK) void k_post(DevMesh m, Params P, const Ctrl*
 // k_post<POST>, so the results are bit for bit theirs.  REC: the recording step of the adjoint,
 // which also keeps X1, and for POST 2 the aux before the source (A3) and the source's solution
 // (X3), as k_post_record does, and the step's dt in dt_log.  Not for the well-balanced cell term.
-template <int MAXF, int POST, bool MC, bool REC>
+template <int MAXF, int POST, bool MC, bool REC, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void k_flux_post(DevMesh m, Params P, const Ctrl* __restrict__ ctrl, int stage,
                                                      const T* __restrict__ QF, const T* __restrict__ As,
                                                      const T* __restrict__ W, const T* X0, T* Xout, T* A,
@@ -1343,24 +1377,25 @@ __global__ __launch_bounds__(BLOCK) void k_flux_post(DevMesh m, Params P, const 
             const T time = ctrl->stage_t[stage];
             T fc[NQ], ac[NA];
             load_soa<NA>(As, ld, c, ac);
-            flux_cell_gather<MAXF>(m, P, c, time, ac, 0, nullptr, QF, As, fc, G);
+            flux_cell_gather<MAXF, NT>(m, P, c, time, ac, 0, nullptr, QF, As, fc, G);
             // VecAXPBYPCZ(X, (s-1)/s, dt/s, 1/s, W, F) as stage_update<MODE_FINAL>
             const T al = ctrl->al, be = ctrl->be, ga = ctrl->ga;
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) q[i] = al * W[(size_t)i * ld + c] + be * fc[i] + ga * X0[(size_t)i * ld + c];
+            for (int i = 0; i < NQ; ++i)
+                q[i] = al * ldx<NT>(W + (size_t)i * ld + c) + be * fc[i] + ga * ldx<NT>(X0 + (size_t)i * ld + c);
         }
-        if constexpr (REC) store_soa<NQ>(X1, ld, c, q);
-        load_soa<NA>(A, ld, c, a);
+        if constexpr (REC) store_soa_x<NT, NQ>(X1, ld, c, q);
+        load_soa_x<NT, NA>(A, ld, c, a);
         update_state_cell(P, q, a);
         if constexpr (POST == 2) {
-            if constexpr (REC) store_soa<NA>(A3, ld, c, a);
+            if constexpr (REC) store_soa_x<NT, NA>(A3, ld, c, a);
             newton_cell(P, ctrl->dt, q, a);
-            if constexpr (REC) store_soa<NQ>(X3, ld, c, q);
+            if constexpr (REC) store_soa_x<NT, NQ>(X3, ld, c, q);
             update_state_cell(P, q, a);
         }
-        store_soa<NQ>(Xout, ld, c, q);
-        store_soa<NA>(A, ld, c, a);
-        return cfl_ratio(P, q, a, m.rc[c]);
+        store_soa_x<NT, NQ>(Xout, ld, c, q);
+        store_soa_x<NT, NA>(A, ld, c, a);
+        return cfl_ratio(P, q, a, ldx<NT>(m.rc + c));
     };
     if constexpr (MC) {
         for (int j = 0; j < m.cpt; ++j) {
@@ -1728,6 +1763,8 @@ struct zdm_solver {
     double kt_ms[4];
     int kt_n[4];
     bool fuse_post = true;            // zdm_fuse_post: the final stage and the post-step as k_flux_post
+    int flux_nt_mode = -1;            // zdm_flux_nontemporal: -1 by size, 0 off, 1 on
+    bool flux_nt = false;             // the stage / final flux launches use their non-temporal form
     hipEvent_t ev_caller = nullptr;   // orders the device-pointer entry points on a caller's stream
     int alpha_mode = 0;               // prep: 0 as configured, 1 store every face's limiter value at
     T* alpha_io = nullptr;            // alpha_io, 2 take them from it (advanced one stage per call)
@@ -1896,6 +1933,13 @@ void launch_flux_t(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T
                                                                 grad_buffer(s));
             break;
         case MODE_STAGE:
+            if constexpr (!MC) {
+                if (s->flux_nt) {
+                    k_flux<MAXF, MODE_STAGE, false, T, true><<<g, b, 0, s->stream>>>(
+                        s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0, grad_buffer(s));
+                    break;
+                }
+            }
             k_flux<MAXF, MODE_STAGE, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
                                                                 grad_buffer(s));
             break;
@@ -1904,6 +1948,13 @@ void launch_flux_t(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T
                                                                 grad_buffer(s));
             break;
         default:
+            if constexpr (!MC) {
+                if (s->flux_nt) {
+                    k_flux<MAXF, MODE_FINAL, false, T, true><<<g, b, 0, s->stream>>>(
+                        s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0, grad_buffer(s));
+                    break;
+                }
+            }
             k_flux<MAXF, MODE_FINAL, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
                                                                 grad_buffer(s));
             break;
@@ -2063,21 +2114,37 @@ void flux_post_t(zdm_solver* s, const Ctrl* ctrl, int mode, const T* QF, const T
     dim3 g(s->nblocks), b(s->bs);
     const int stage = s->opt.ssp_stages - 1;
     const T* G = grad_buffer(s);
-#define ZDM_FP(PO, RE)                                                                                       \
-    k_flux_post<MAXF, PO, MC, RE><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, W, X0, Xout, A,  \
-                                                          s->partials, G, rec ? rec->X1 : nullptr,            \
-                                                          rec ? rec->X3 : nullptr, rec ? rec->A3 : nullptr,   \
-                                                          rec ? rec->dt_log : nullptr)
+#define ZDM_FP(PO, RE, NT)                                                                                    \
+    k_flux_post<MAXF, PO, MC, RE, NT><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, stage, QF, As, W, X0, Xout, A, \
+                                                              s->partials, G, rec ? rec->X1 : nullptr,         \
+                                                              rec ? rec->X3 : nullptr, rec ? rec->A3 : nullptr, \
+                                                              rec ? rec->dt_log : nullptr)
+    // the non-temporal form for the one-cell-per-thread shape only (what a large mesh autotunes to)
+    const bool nt = !MC && s->flux_nt;
     if (mode == 2) {
-        if (rec)
-            ZDM_FP(2, true);
-        else
-            ZDM_FP(2, false);
+        if (rec) {
+            if (nt)
+                ZDM_FP(2, true, !MC);
+            else
+                ZDM_FP(2, true, false);
+        } else {
+            if (nt)
+                ZDM_FP(2, false, !MC);
+            else
+                ZDM_FP(2, false, false);
+        }
     } else {
-        if (rec)
-            ZDM_FP(1, true);
-        else
-            ZDM_FP(1, false);
+        if (rec) {
+            if (nt)
+                ZDM_FP(1, true, !MC);
+            else
+                ZDM_FP(1, true, false);
+        } else {
+            if (nt)
+                ZDM_FP(1, false, !MC);
+            else
+                ZDM_FP(1, false, false);
+        }
     }
 #undef ZDM_FP
 }
@@ -2262,6 +2329,19 @@ int zdm_model_parameter_names(char* names64, int cap) {
     return (int)v.size();
 }
 
+// By size: the non-temporal flux form where one flux pass (48 n + 48 B per cell and 40 B per
+// face) is larger than the 256 MB last-level cache.  Measured at C2 (2 M cells, 0.98 GB per
+// pass): 2591 - 2597 -> 2616 - 2620 Mcell-steps/s; at C5 (250 k cells, 0.10 GB) the forward
+// was 4 % slower with it.
+static void choose_flux_nt(zdm_solver* s) {
+    if (s->flux_nt_mode >= 0) {
+        s->flux_nt = s->flux_nt_mode != 0;
+        return;
+    }
+    const double pass = (double)s->ncells * (48.0 * NQ + 48.0) + 40.0 * (double)s->nfaces;
+    s->flux_nt = pass > 256.0 * 1024.0 * 1024.0;
+}
+
 static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* params,
                        const zdm_options* opts, int nmem, const int* cell_off, zdm_solver** out) {
     if (!mesh || !face_bc || !opts || !out) return fail("zdm_solver_create: null argument");
@@ -2313,6 +2393,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
     const int nc = mesh->ncells, nf = mesh->nfaces, MF = mesh->max_faces;
     s->ncells = nc;
     s->nfaces = nf;
+    choose_flux_nt(s);
     s->maxf = MF;
     s->ld = (int)align_up((size_t)nc, 256);
     s->nmem = nmem;
@@ -2863,6 +2944,19 @@ int zdm_enable_timing(zdm_solver* s, int on) {
 int zdm_fuse_post(zdm_solver* s, int on) {
     if (!s) return fail("null solver");
     s->fuse_post = on != 0;
+    return 0;
+}
+
+int zdm_flux_nontemporal(zdm_solver* s, int mode, int* active) {
+    if (!s) return fail("null solver");
+    if (mode < -1 || mode > 1) return fail("zdm_flux_nontemporal: mode must be -1, 0 or 1");
+    s->flux_nt_mode = mode;
+    choose_flux_nt(s);
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);   // a captured step holds the old kernels
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+    if (active) *active = s->flux_nt ? 1 : 0;
     return 0;
 }
 
