@@ -69,7 +69,8 @@ def step_bytes_per_cell(n_q, faces_per_cell, stages=2):
 
 
 def pmc_traffic(kernel, cells):
-    """HBM bytes per launch of `kernel` (k_flux: the mean of its stage and final launches) from
+    """Bytes per launch of `kernel` past the L2 (an upper bound on its HBM bytes; k_flux: the mean of
+    its stage and final launches) from
     the committed FETCH_SIZE / WRITE_SIZE profile of the same workload
     (profiles/k_flux_c2_2M_pmc.md, copy-kernel calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "k_flux_c2_2M_pmc.md")
@@ -1006,7 +1007,8 @@ def main():
                   "stage fused with the post-step: constraints, Newton source, constraints, CFL)"
         traffic = pmc_traffic("k_flux+k_flux_post", ncells)
         tsrc = "rocprofv3 FETCH_SIZE + WRITE_SIZE, mean of the k_flux and k_flux_post launches, " \
-               "copy-kernel calibrated (profiles/k_flux_c2_2M_pmc.md)"
+               "copy-kernel calibrated: bytes past the L2, served by the Infinity Cache or HBM, so an " \
+               "upper bound on HBM bytes (profiles/k_flux_c2_2M_pmc.md)"
     else:
         nl, bpc, launch_ms = flux_n, flux_launch_bytes_per_cell(info.n_q, F), flux_ms
         rkernel = "k_flux (flux + fluctuations gather + SSP stage update)"
