@@ -161,10 +161,11 @@ int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches);
  * reference (its MUSCLSolver runs TSStep, then the source and the constraints, as separate
  * passes: MUSCLSolver.hpp:140-188). */
 int zdm_fuse_post(zdm_solver* s, int on);
-/* Cache policy of the stage and final flux launches (one cell per thread): mode 1 loads and stores
- * their read-once and write-once streams (stage state, Q^n, output state and aux, the cell's own
- * slot topology and volume) non-temporally, so that they do not displace the face states and face
- * records that neighbours read again in the same launch; 0 uses ordinary accesses; -1 (the
+/* Cache policy of the transport kernels (one cell per thread): mode 1 loads and stores their
+ * read-once and write-once streams non-temporally (in the stage and final flux launches the stage
+ * state, Q^n, output state and aux, the cell's own slot topology and volume; in the Green-Gauss
+ * reconstruction the cell's own slot topology, slot geometry and volume), so that they do not
+ * displace data that neighbours or the next launch read again; 0 uses ordinary accesses; -1 (the
  * default) picks 1 where one flux pass (48 n_q + 48 B per cell, 40 B per face) exceeds 256 MiB,
  * the last-level cache.  *active (may be null) receives the form now in use.  Results are bitwise
  * the same either way.  No counterpart in the reference (a cache policy of this build's kernels). */

@@ -321,13 +321,16 @@ __device__ __forceinline__ void store_soa(T* __restrict__ a, int ld, int c, cons
 #pragma unroll
     for (int i = 0; i < N; ++i) a[(size_t)i * ld + c] = v[i];
 }
-// The flux kernels' read-once and write-once streams (the stage state, Q^n, the output state
-// and aux, the cell's own slot topology and volume), non-temporal when NT: on a mesh whose flux
-// pass is far larger than the last-level cache they then do not displace the face states and
-// face records that a cell's neighbours read again in the same launch (C2: 551 -> 512 B/cell
-// moved by the stage launch).  On a smaller mesh the same hints keep data out of the 256 MB
-// Infinity Cache that the next kernel would have found there, and cost time, so the solver
-// picks them by size (zdm_flux_nontemporal).
+// The transport kernels' read-once and write-once streams, non-temporal when NT: in the flux
+// kernels the stage state, Q^n, the output state and aux, the cell's own slot topology and
+// volume; in the Green-Gauss k_prep the cell's own slot topology, slot geometry and volume.
+// On a mesh whose flux pass is far larger than the last-level cache they then do not displace
+// the data that neighbours read again in the same launch (C2: 551 -> 512 B/cell moved by the
+// stage flux launch), nor the face states that the next flux launch reads from the Infinity
+// Cache (C2: k_prep 0.166 -> 0.162 ms).  On a smaller mesh the same hints keep data out of the
+// 256 MB Infinity Cache that the next kernel or step would have found there, and cost time, so
+// the solver picks them by size (zdm_flux_nontemporal).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 template <bool NT, typename S>
 __device__ __forceinline__ S ldx(const S* p) {
     if constexpr (NT)
@@ -377,7 +380,7 @@ __device__ __forceinline__ void update_state_cell(const Params& P, T* q, T* a) {
 // wet_eps / zs_theta: GRAD 3 only (wet_dry_eps; 1 = apply the Zhang-Shu theta).
 // AMODE (limited linear reconstruction, GRAD 1 / 2): 0 computes the face limiter values alpha,
 // 1 computes and stores them in abuf[k ld + c], 2 takes them from abuf (a frozen limiter)
-template <int MAXF, int GRAD, typename Put, int AMODE = 0>
+template <int MAXF, int GRAD, typename Put, int AMODE = 0, bool NT = false>
 __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, const T* __restrict__ X,
                                            const T* __restrict__ Ag, int c, int use_limiters, T* ac, Put put,
                                            T wet_eps = 0, int zs_theta = 0, T* abuf = nullptr, T* gout = nullptr) {
@@ -406,8 +409,8 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
         int fk[MAXF], nbk[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
-            fk[k] = m.cface[(size_t)k * ld + c];
-            nbk[k] = m.cnbr[(size_t)k * ld + c];
+            fk[k] = ldx<NT>(m.cface + (size_t)k * ld + c);
+            nbk[k] = ldx<NT>(m.cnbr + (size_t)k * ld + c);
         }
         T qn[MAXF][NQ], an[MAXF][NA], Nk[MAXF][2], dxk[MAXF][2];
 #pragma unroll
@@ -415,8 +418,8 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
             const int nbs = nbk[k] >= 0 ? nbk[k] : c;
             load_soa<NQ>(X, ld, nbs, qn[k]);
             load_soa<NA>(Ag, ld, nbs, an[k]);
-            const double2 ns = m.sgeo[sgeo_idx(ld, k, 0, c)];
-            const double2 dx = m.sgeo[sgeo_idx(ld, k, 1, c)];
+            const dvec2 ns = ldx<NT>(reinterpret_cast<const dvec2*>(m.sgeo + sgeo_idx(ld, k, 0, c)));
+            const dvec2 dx = ldx<NT>(reinterpret_cast<const dvec2*>(m.sgeo + sgeo_idx(ld, k, 1, c)));
             Nk[k][0] = ns.x;
             Nk[k][1] = ns.y;
             dxk[k][0] = dx.x;
@@ -478,7 +481,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
             }
         }
         if constexpr (GG) {
-            const T vol = m.vol[c];
+            const T vol = ldx<NT>(m.vol + c);
             if (vol > 1e-15) {
 #pragma unroll
                 for (int i = 0; i < NQ * DIM; ++i) g[i] /= vol;
@@ -638,7 +641,7 @@ __device__ __forceinline__ void recon_cell(const DevMesh& m, const Params& P, co
     }
 }
 
-template <int MAXF, int GRAD, bool MC>
+template <int MAXF, int GRAD, bool MC, bool NT = false>
 __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ctrl* __restrict__ ctrl,
                                                 const T* __restrict__ X, const T* __restrict__ Ag,
                                                 T* __restrict__ As, T* __restrict__ QF, int use_limiters,
@@ -649,9 +652,9 @@ __global__ __launch_bounds__(BLOCK, 3) void k_prep(DevMesh m, Params P, const Ct
     const int ld = m.ld;
     auto body = [&](int c) {
         T ac[NA];
-        recon_cell<MAXF, GRAD>(
-            m, P, X, Ag, c, use_limiters, ac, [&](int k, int i, T v) { QF[qf_idx(ld, k, c, i)] = v; },
-            wet_eps, zs_theta, nullptr, G);
+        auto put = [&](int k, int i, T v) { QF[qf_idx(ld, k, c, i)] = v; };
+        recon_cell<MAXF, GRAD, decltype(put), 0, NT>(m, P, X, Ag, c, use_limiters, ac, put, wet_eps, zs_theta, nullptr,
+                                                      G);
         store_soa<NA>(As, ld, c, ac);
     };
     if constexpr (MC) {
@@ -1913,6 +1916,9 @@ void launch_prep_t(zdm_solver* s, const Ctrl* ctrl, const T* Xin, const T* Ag, T
     else if (s->opt.gradient == 1)
         k_prep<MAXF, 2, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0,
                                                            grad_buffer(s));
+    else if (!MC && s->flux_nt)
+        k_prep<MAXF, 1, false, true><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0,
+                                                             grad_buffer(s));
     else
         k_prep<MAXF, 1, MC><<<g, b, 0, s->stream>>>(s->dm, s->P, ctrl, Xin, Ag, As, QF, lim, eps, 0,
                                                            grad_buffer(s));
