@@ -227,7 +227,7 @@ struct DevMesh {
     const T* fnhy;
     const int* fbc;   // model BC index of boundary faces, -1 = none
     // k_flux's packed copies: ctopo[2 (k ld + c)] = cface, [.. + 1] = cnbr | cmeta << 28
-    // (-1 without a neighbour); fgeo[4 f] = (fnhx, fnhy, farea, 0)
+    // (-1 without a neighbour); fgeo[3 f] = (fnhx, fnhy, farea)
     const int* ctopo;
     const T* fgeo;
     // per face slot of each cell, two 16-byte records (sgeo_idx): the area-weighted normal
@@ -914,7 +914,7 @@ __device__ __forceinline__ void flux_cell_gather(const DevMesh& m, const Params&
         const int f = fk[k] >= 0 ? fk[k] : 0;
         const int nb = nk[k] >= 0 ? nk[k] : c;
         const int ks = nk[k] >= 0 ? (mk[k] & 3) : 0;
-        const T* g = m.fgeo + 4 * (size_t)f;
+        const T* g = m.fgeo + 3 * (size_t)f;
         in.nh[0] = g[0];
         in.nh[1] = DIM > 1 ? g[1] : (T)0;
         in.area = g[2];
@@ -2438,7 +2438,7 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
         return fail("mesh too large: the packed neighbour index holds fewer than 2^28 cells");
     }
     std::vector<int> ctopo((size_t)2 * MFa * ld, -1);
-    std::vector<T> fgeo((size_t)4 * std::max(nf, 1), 0.0);
+    std::vector<T> fgeo((size_t)3 * std::max(nf, 1), 0.0);
     std::vector<double2> sgeo((size_t)MFa * 2 * ld, make_double2(0.0, 0.0));
     for (int f = 0; f < nf; ++f) {
         const double* N = mesh->face_normal + 3 * (size_t)f;
@@ -2453,9 +2453,9 @@ static int create_impl(const zdm_mesh* mesh, const int* face_bc, const double* p
         fnhx[f] = N[0] / area;
         fnhy[f] = DIM > 1 ? N[1] / area : 0;
         fbc[f] = mesh->face_cells[2 * f + 1] >= 0 ? -1 : face_bc[f];
-        fgeo[4 * (size_t)f] = fnhx[f];
-        fgeo[4 * (size_t)f + 1] = fnhy[f];
-        fgeo[4 * (size_t)f + 2] = area;
+        fgeo[3 * (size_t)f] = fnhx[f];
+        fgeo[3 * (size_t)f + 1] = fnhy[f];
+        fgeo[3 * (size_t)f + 2] = area;
     }
     for (int c = 0; c < nc; ++c) {
         vol[c] = mesh->cell_volume[c];
