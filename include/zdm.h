@@ -162,8 +162,8 @@ int zdm_kernel_time(zdm_solver* s, int kernel, double* avg_ms, int* launches);
  * passes: MUSCLSolver.hpp:140-188). */
 int zdm_fuse_post(zdm_solver* s, int on);
 /* Cache policy of the transport kernels (one cell per thread): mode 1 loads and stores their
- * read-once and write-once streams non-temporally (in the stage and final flux launches the stage
- * state, Q^n, output state and aux, the cell's own slot topology and volume; in the Green-Gauss
+ * read-once and write-once streams non-temporally (in the flux launches the stage state, Q^n, the
+ * output state and aux, the cell's own slot topology and volume; in the Green-Gauss
  * reconstruction the cell's own slot topology, slot geometry and volume), so that they do not
  * displace data that neighbours or the next launch read again; 0 uses ordinary accesses; -1 (the
  * default) picks 1 where one flux pass (48 n_q + 48 B per cell, 40 B per face) exceeds 256 MiB,

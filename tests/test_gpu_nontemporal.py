@@ -99,3 +99,26 @@ def test_nontemporal_form_in_the_adjoint_recording_step():
     assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2][0], out[1][2][0]) and np.array_equal(out[0][2][1], out[1][2][1])
     assert np.abs(out[0][2][0]).max() > 0
+
+
+@pytest.mark.parametrize("ti", ["imex", "implicit"])
+def test_nontemporal_form_in_imex_and_implicit_steps(ti):
+    """The right-hand-side and IMEX flux launches (k_flux MODE_RHS / MODE_IMEX3) in both forms."""
+    mesh = zdm.Mesh.tri_grid(16, 16, jitter=0.15, seed=31)
+    X = cases.c2_state(mesh, 8, seed=31)
+    rng = np.random.default_rng(31)
+    X[:, 2:] = 0.05 * rng.standard_normal((mesh.ncells, 6))
+    out = []
+    for nt in (1, 0):
+        print("case sme2", ti, "nontemporal", nt, flush=True)
+        kw = dict(use_limiters=0) if ti == "implicit" else {}
+        s = zdm.Solver("sme2", mesh, order=2, ssp_stages=2, cfl=0.4, t_end=1e9, source=True, time_integration=ti,
+                       **kw)
+        assert s.flux_nontemporal(nt) == bool(nt)
+        s.upload(X, np.zeros((mesh.ncells, 1)))
+        s.initialize()
+        s.step(4)
+        Xg, Ag = s.download()
+        out.append((Xg, Ag, s.time()))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]) and out[0][2] == out[1][2]

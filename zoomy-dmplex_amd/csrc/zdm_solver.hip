@@ -1941,6 +1941,13 @@ void launch_flux_t(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T
     const int wb = s->opt.well_balanced;
     switch (mode) {
         case MODE_RHS:
+            if constexpr (!MC) {
+                if (s->flux_nt) {
+                    k_flux<MAXF, MODE_RHS, false, T, true><<<g, b, 0, s->stream>>>(
+                        dmr, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0, grad_buffer(s));
+                    break;
+                }
+            }
             k_flux<MAXF, MODE_RHS, MC><<<g, b, 0, s->stream>>>(dmr, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
                                                                 grad_buffer(s));
             break;
@@ -1960,6 +1967,13 @@ void launch_flux_t(zdm_solver* s, const Ctrl* ctrl, int mode, int stage, const T
                                                                 grad_buffer(s));
             break;
         case MODE_IMEX3:
+            if constexpr (!MC) {
+                if (s->flux_nt) {
+                    k_flux<MAXF, MODE_IMEX3, false, T, true><<<g, b, 0, s->stream>>>(
+                        dmr, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0, grad_buffer(s));
+                    break;
+                }
+            }
             k_flux<MAXF, MODE_IMEX3, MC><<<g, b, 0, s->stream>>>(dmr, s->P, ctrl, stage, QF, As, In, Out, Ag, wb, X0,
                                                                 grad_buffer(s));
             break;
